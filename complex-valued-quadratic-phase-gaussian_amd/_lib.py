@@ -1,0 +1,73 @@
+"""ctypes binding of libcvqpg.so (the C ABI declared in include/cvqpg.h).
+
+torch is imported first on purpose: it loads its bundled HIP runtime
+(libamdhip64.so.7), so the library's own dependency on that SONAME resolves to
+the same runtime instance and both share devices, streams and allocations.
+
+There is no fallback: if the library is missing or fails to load, every entry
+point raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcvqpg.so")
+
+_c_int, _c_i64, _c_f32, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
+_p = ctypes.c_void_p
+
+# name -> (restype, argtypes); must match include/cvqpg.h
+SIGNATURES = {
+    "cvqpg_last_error": (ctypes.c_char_p, []),
+    "cvqpg_abi_version": (_c_int, []),
+    "cvqpg_raster_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_i64]),
+    "cvqpg_bin_count": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _c_i64, _p, _p]),
+    "cvqpg_bin_sort": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _c_i64, _p, _p, _p]),
+    "cvqpg_render_forward": (_c_int, [_p] * 8 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p,
+                                                 _c_size, _c_i64, _p, _p, _p, _p]),
+    "cvqpg_render_backward": (_c_int, [_p] * 9 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p,
+                                                  _c_size, _c_i64, _p, _p] + [_p] * 7 + [_p]),
+    "cvqpg_qpf_render_forward": (_c_int, [_p] * 9 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p,
+                                                     _c_size, _c_i64, _p, _p, _p, _p, _p]),
+}
+
+_LIB = None
+
+
+class CvqpgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load (once) and return the ctypes handle; raises if the library is absent."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CvqpgError(f"{LIB_PATH} is not built; run build.py (hipcc, gfx950)")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def loaded_path():
+    return LIB_PATH if _LIB is not None else None
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().cvqpg_last_error().decode(errors="replace")
+        raise CvqpgError(msg or f"libcvqpg error {rc}")
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,174 @@
+"""Drop-in replacement for the reference's cuda/rasterizer.py on MI355X.
+
+``qpf_render`` keeps the signature, argument meaning, return type and error
+behaviour of cuda/rasterizer.py:62-71 (``QPFRender`` of :36-59): inputs are the
+activated per-Gaussian tensors, the result is the complex64 field [C, H, W], and
+the backward returns gradients for the first seven arguments and ``None`` for
+radii, k, dxy2, width and height.  The work is done by libcvqpg (hand-written
+gfx950 HIP kernels) through its C ABI; there is no CPU or PyTorch fallback.
+
+Like cuda/raster.cu:71 the eager forward reads the key count P back to the host
+once to size its key buffers; the graph-captured training step uses
+``cvqpg_qpf_render_forward`` with a pre-sized capacity instead.
+"""
+import math
+
+import torch
+from torch.autograd import Function
+
+from . import _lib
+
+TILE = 16
+_capacity_hint = {}
+
+
+def _check_input(t, name, n):
+    """cuda/raster.cu:22-28: device tensor, contiguous, float32/int32, first dim n."""
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be CUDA")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+    if t.dtype not in (torch.float32, torch.int32):
+        raise RuntimeError(f"{name} must be float32/int32")
+    if t.size(0) != n:
+        raise RuntimeError(f"{name} has {t.size(0)} rows, expected {n}")
+
+
+def workspace_bytes(n, channels, width, height, capacity):
+    return int(_lib.lib().cvqpg_raster_workspace_size(n, channels, width, height, capacity))
+
+
+class Binning:
+    """Key list of one forward pass (sorted ids, tile ranges, workspace)."""
+
+    def __init__(self, ws, capacity, point_list, ranges, num_pairs):
+        self.ws, self.capacity = ws, capacity
+        self.point_list, self.ranges, self.num_pairs = point_list, ranges, num_pairs
+
+
+def bin_gaussians(means_2d, radii, width, height, channels=3):
+    """Count, emit, sort and range the (tile, Gaussian) keys on the GPU."""
+    L = _lib.lib()
+    n = means_2d.shape[0]
+    dev = means_2d.device
+    key = (n, width, height)
+    cap = max(1024, int(_capacity_hint.get(key, 16 * max(n, 1)) * 1.125))
+    stream = _lib.stream_handle(dev)
+    num = torch.empty(1, dtype=torch.int32, device=dev)
+    ws = torch.empty(workspace_bytes(n, channels, width, height, cap), dtype=torch.uint8, device=dev)
+    _lib.check(L.cvqpg_bin_count(_lib.ptr(means_2d), _lib.ptr(radii), n, width, height,
+                                 _lib.ptr(ws), ws.numel(), cap, _lib.ptr(num), stream))
+    P = int(num.item())  # host read-back of P, as cuda/raster.cu:71
+    if P > cap:
+        cap = P
+        ws = torch.empty(workspace_bytes(n, channels, width, height, cap), dtype=torch.uint8,
+                         device=dev)
+        _lib.check(L.cvqpg_bin_count(_lib.ptr(means_2d), _lib.ptr(radii), n, width, height,
+                                     _lib.ptr(ws), ws.numel(), cap, _lib.ptr(num), stream))
+    _capacity_hint[key] = P
+    T = math.ceil(width / TILE) * math.ceil(height / TILE)
+    point_list = torch.empty(max(P, 1), dtype=torch.int32, device=dev)
+    ranges = torch.empty(T, 2, dtype=torch.int32, device=dev)
+    _lib.check(L.cvqpg_bin_sort(_lib.ptr(means_2d), _lib.ptr(radii), n, width, height,
+                                _lib.ptr(ws), ws.numel(), cap, _lib.ptr(point_list),
+                                _lib.ptr(ranges), stream))
+    return Binning(ws, cap, point_list, ranges, P)
+
+
+def render_forward(means_2d, inv_cov, colours, phase, opacities, curvature, area, k, dxy2,
+                   width, height, bins):
+    n, C = colours.shape
+    field = torch.empty(C, height, width, 2, dtype=torch.float32, device=means_2d.device)
+    _lib.check(_lib.lib().cvqpg_render_forward(
+        _lib.ptr(means_2d), _lib.ptr(inv_cov), _lib.ptr(colours), _lib.ptr(phase),
+        _lib.ptr(opacities), _lib.ptr(curvature), _lib.ptr(area), _lib.ptr(k), n, C, float(dxy2),
+        width, height, _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity,
+        _lib.ptr(bins.point_list), _lib.ptr(bins.ranges), _lib.ptr(field),
+        _lib.stream_handle(means_2d.device)))
+    return torch.view_as_complex(field)
+
+
+def render_backward(grad_field, means_2d, inv_cov, colours, phase, opacities, curvature, area, k,
+                    dxy2, width, height, bins):
+    n, C = colours.shape
+    dev = means_2d.device
+    g = torch.view_as_real(grad_field.resolve_conj().contiguous()).contiguous()
+    outs = [torch.empty(n, 2, device=dev), torch.empty(n, 3, device=dev),
+            torch.empty(n, C, device=dev), torch.empty(n, C, device=dev),
+            torch.empty(n, device=dev), torch.empty(n, device=dev), torch.empty(n, device=dev)]
+    if n == 0:
+        return [o.zero_() for o in outs]
+    _lib.check(_lib.lib().cvqpg_render_backward(
+        _lib.ptr(g), _lib.ptr(means_2d), _lib.ptr(inv_cov), _lib.ptr(colours), _lib.ptr(phase),
+        _lib.ptr(opacities), _lib.ptr(curvature), _lib.ptr(area), _lib.ptr(k), n, C, float(dxy2),
+        width, height, _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity,
+        _lib.ptr(bins.point_list), _lib.ptr(bins.ranges), *[_lib.ptr(o) for o in outs],
+        _lib.stream_handle(dev)))
+    return outs
+
+
+class QPFRender(Function):
+    """Autograd wrapper with the signature of cuda/rasterizer.py:36-59."""
+
+    @staticmethod
+    def forward(ctx, means_2d, inv_cov, colours, phase, opacities, curvature, area,
+                radii, k, dxy2, width, height):
+        C = colours.shape[1]
+        if C not in (1, 3):
+            raise RuntimeError(f"qpf rasterizer supports 1 or 3 channels, got {C}")
+        args = [t.contiguous() for t in (means_2d, inv_cov, colours, phase, opacities,
+                                         curvature, area)]
+        n = args[0].shape[0]
+        radii_i = radii.contiguous().int()
+        for t, name in zip(args + [radii_i], ("means_2d", "inv_cov", "colours", "phase",
+                                                "opacities", "curvature", "area", "radii")):
+            _check_input(t, name, n)
+        kk = k.contiguous().float()
+        if kk.numel() != C:
+            raise RuntimeError("k must have num_channels entries")
+        bins = bin_gaussians(args[0], radii_i, int(width), int(height), C)
+        field = render_forward(*args, kk, dxy2, int(width), int(height), bins)
+        ctx.save_for_backward(*args, kk, bins.point_list, bins.ranges, bins.ws)
+        ctx.cfg = (float(dxy2), int(width), int(height), bins.capacity, bins.num_pairs)
+        return field
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (means, inv_cov, colours, phase, opac, curv, area, k, point_list, ranges,
+         ws) = ctx.saved_tensors
+        dxy2, W, H, cap, P = ctx.cfg
+        bins = Binning(ws, cap, point_list, ranges, P)
+        d = render_backward(grad_out, means, inv_cov, colours, phase, opac, curv, area, k, dxy2,
+                            W, H, bins)
+        return d[0], d[1], d[2], d[3], d[4], d[5], d[6], None, None, None, None, None
+
+
+def qpf_render(means_2d, inv_cov, colours, phase, opacities, curvature, area,
+               radii, k, dxy2, width, height):
+    """Complex field [C, H, W] of N Gaussians (linear coherent sum, quadratic phase).
+
+    means_2d [N,2] pixel coords, inv_cov [N,3] = (inv00, inv01, inv11), colours/phase [N,C],
+    opacities/curvature/area [N] (area = |sx*sy|), radii [N] culling radius in px,
+    k [C] wavenumbers, dxy2 = pixel_pitch**2.
+    """
+    return QPFRender.apply(means_2d, inv_cov, colours, phase, opacities, curvature, area,
+                           radii, k, dxy2, width, height)
+
+
+def qpf_render_reference(means_2d, inv_cov, colours, phase, opacities, curvature, area,
+                         k, dxy2, width, height):
+    """Dense all-pairs evaluation in PyTorch (no culling), the testing helper that the
+    reference exports next to qpf_render (cuda/rasterizer.py:74-90).  O(N*H*W) memory:
+    small inputs only; never used by qpf_render."""
+    dev = means_2d.device
+    C = colours.shape[1]
+    ys, xs = torch.meshgrid(torch.arange(height, device=dev, dtype=means_2d.dtype),
+                            torch.arange(width, device=dev, dtype=means_2d.dtype), indexing="ij")
+    d = means_2d[:, None, :] - torch.stack([xs, ys], -1).reshape(1, -1, 2)
+    dx, dy = d[..., 0], d[..., 1]
+    m = inv_cov[:, 0:1] * dx * dx + 2 * inv_cov[:, 1:2] * dx * dy + inv_cov[:, 2:3] * dy * dy
+    alpha = opacities[:, None] * torch.exp(torch.clamp(-0.5 * m, min=-50.0))
+    base = 0.5 * curvature[:, None] * m * area[:, None] * dxy2
+    ph = phase.t()[..., None] + k.view(C, 1, 1) * base[None]
+    amp = colours.t()[..., None] * alpha[None]
+    return torch.polar(amp, ph).sum(1).view(C, height, width)
