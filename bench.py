@@ -1,0 +1,152 @@
+"""Training-step throughput of the CVQPG hot path on one MI355X.
+
+Workload (the metric's configuration): RGB 1920x1080, 150,000 Gaussians, FP32 /
+complex64 field, 3 wavelengths (639/515/473 nm), 8 um pitch, 2x padded BLASM
+(2816 x 4480 FFT grid) to 4 planes at 0, 2, 4, 6 mm; one step = rasterize -> pad ->
+glue -> BLASM -> multi-plane L2 + SSIM loss -> full adjoint -> Adan.
+
+Prints one JSON line (rank 0).  ``roofline`` describes the dominant kernel, timed
+with HIP events on its launch stream inside the timed region.  ``cpu_baseline``
+times the CPU oracle on a bounded sample of the same workload (see
+``cpu_baseline``).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "train step/s @1080p RGB 150k Gaussians; time-to-PSNR warm-start vs plain"
+WORKLOAD = "rgb1080p_150k"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", default="auto", choices=["auto", "autograd"],
+                    help="auto: fastest available step; autograd: reference-structured step")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default=WORKLOAD)
+    return ap.parse_args()
+
+
+def raster_flops_per_pair(C):
+    """Algorithmic FP32 operations per (pixel, Gaussian) pair of the reference math
+    (cuda/forward.cu:131-151, cuda/backward.cu:101-148), counting each add, mul,
+    compare and transcendental as one operation and an FMA as two."""
+    fwd = 17 + 9 * C          # offsets 2, mahal 9, clamp 2, exp 1, alpha 1, base 1 ; per channel 9
+    bwd = 46 + 19 * C
+    return fwd, bwd
+
+
+def cpu_baseline(spec, seed=0):
+    """CPU oracle on a bounded sample of the workload, scaled to step/s.
+
+    Sample: (a) the reference's PyTorch tile renderer (3 sigma, 64-px tiles,
+    oracle.model.render_tile_torch) forward + autograd backward in FP32 on a
+    256 x 256 window holding the Gaussians whose centres fall inside it, scaled by
+    the frame/window area ratio; (b) one forward + one inverse complex64 2D FFT of
+    the full 2816 x 4480 grid (torch.fft / pocketfft), scaled by the 4 D C FFTs of
+    the reference step (forward and autograd backward).  Elementwise work of the
+    step is not counted, so the baseline is optimistic for the CPU."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from oracle import model as omodel
+    import __graft_entry__ as ge
+    pkg = ge.load_pkg()
+    from cvqpg_amd import synthetic as syn
+    W, H, N = spec["W"], spec["H"], spec["N"]
+    C = len(spec["wavelengths"])
+    raw = syn.make_scene(N, W, H, spec["wavelengths"], spec["pitch"], spec["kappa"], seed)
+    win = 512
+    tr = {k: torch.as_tensor(v) for k, v in raw.items()}
+    mt = torch.tanh(tr["means_2d"])
+    cx, cy = (mt[:, 0] + 1) * 0.5 * W, (mt[:, 1] + 1) * 0.5 * H
+    sel = (cx < win) & (cy < win)
+    sub = {k: v[sel].clone().requires_grad_() for k, v in tr.items()}
+    threads = torch.get_num_threads()
+    t0 = time.perf_counter()
+    ri = omodel.raster_inputs(sub, (W, H), spec["wavelengths"], spec["pitch"])
+    out = omodel.render_tile_torch(ri, win, win)
+    (out.abs() ** 2).sum().backward()
+    t_r = time.perf_counter() - t0
+    pH, pW = syn.pad_size_for(W, H)
+    u = torch.randn(2 * pH, 2 * pW, dtype=torch.complex64)
+    t0 = time.perf_counter()
+    torch.fft.ifft2(torch.fft.fft2(u))
+    t_fft = time.perf_counter() - t0
+    D = len(syn.distances_for(spec))
+    t_step = t_r * (W * H) / (win * win) + t_fft * 2 * D * C
+    return {"value": 1.0 / t_step, "unit": "step/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle PyTorch tile renderer fwd+bwd FP32 on a {win}x{win} window "
+                       f"({int(sel.sum())} Gaussians) scaled x{W * H / win / win:.1f}; one "
+                       f"fft2+ifft2 pair of {2 * pH}x{2 * pW} scaled x{2 * D * C}; "
+                       f"elementwise work not counted; {t_r:.1f}s + {t_fft:.1f}s measured")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs "
+              "are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    pkg = ge.load_pkg()
+    from cvqpg_amd import rasterizer, train, synthetic
+
+    dev = torch.device("cuda:0")
+    spec = synthetic.WORKLOADS[args.workload]
+    W, H, N = spec["W"], spec["H"], spec["N"]
+    C = len(spec["wavelengths"])
+    wl = train.Workload(args.workload, device=dev, seed=0)
+    trainer = train.AutogradTrainer(wl, backend="torch")
+    path = "autograd: HIP raster + torch.fft BLASM + torch loss/Adan"
+    for _ in range(args.warmup):
+        trainer.step()
+    torch.cuda.synchronize()
+    rasterizer.timer.enabled = True
+    rasterizer.timer.reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        trainer.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rasterizer.timer.enabled = False
+    ms = 1e3 * dt / args.steps
+    P = rasterizer._capacity_hint.get((N, W, H), 0)
+    pairs = P * 256
+    f_fwd, f_bwd = raster_flops_per_pair(C)
+    k_ms = rasterizer.timer.mean_ms("k_render_bwd")
+    k_fwd_ms = rasterizer.timer.mean_ms("k_render_fwd")
+    achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
+    roof = {"kernel": "k_render_bwd (raster adjoint)", "bound": "mfma", "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None, "avg_ms": k_ms, "pairs": pairs, "flop_per_pair": f_bwd,
+            "k_render_fwd_avg_ms": k_fwd_ms}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(spec)
+    pH, pW = synthetic.pad_size_for(W, H)
+    out = {"metric": METRIC, "value": 1e3 / ms, "unit": "step/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
+                      "channels": C, "planes": len(synthetic.distances_for(spec)),
+                      "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcvqpg.so")
 
 _c_int, _c_i64, _c_f32, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
+_c_f64 = ctypes.c_double
 _p = ctypes.c_void_p
 
 # name -> (restype, argtypes); must match include/cvqpg.h
@@ -31,6 +32,21 @@ SIGNATURES = {
                                                   _c_size, _c_i64, _p, _p] + [_p] * 7 + [_p]),
     "cvqpg_qpf_render_forward": (_c_int, [_p] * 9 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p,
                                                      _c_size, _c_i64, _p, _p, _p, _p, _p]),
+    "cvqpg_raster_pack": (_c_int, [_p] * 8 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p, _c_size,
+                                              _c_i64, _p]),
+    "cvqpg_render_forward_packed": (_c_int, [_c_int, _c_int, _c_int, _c_int, _p, _c_size, _c_i64,
+                                             _p, _p, _p, _p]),
+    "cvqpg_render_backward_tiles": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _p, _c_size,
+                                             _c_i64, _p, _p, _p]),
+    "cvqpg_render_backward_reduce": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_f32, _c_int, _c_int,
+                                              _p, _c_size, _c_i64] + [_p] * 7 + [_p]),
+    "cvqpg_blasm_workspace_size": (_c_size, [_c_int] * 7),
+    "cvqpg_blasm_forward": (_c_int, [_p] + [_c_int] * 6 + [_p, _p, _c_f64, _c_f64, _c_int, _p,
+                                                          _c_size, _p, _p, _p]),
+    "cvqpg_blasm_backward": (_c_int, [_p, _p, _p] + [_c_int] * 6 + [_p, _p, _c_f64, _c_f64, _c_int,
+                                                                    _p, _c_size, _p, _p]),
+    "cvqpg_blasm_reconstruct": (_c_int, [_p, _p] + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64, _c_int,
+                                                                   _p, _c_size, _p, _p]),
 }
 
 _LIB = None
@@ -67,6 +83,12 @@ def check(rc):
 
 def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def doubles(values):
+    """Host array of doubles for the C ABI's small parameter lists."""
+    vals = [float(v) for v in values]
+    return (ctypes.c_double * len(vals))(*vals)
 
 
 def stream_handle(device=None):

@@ -1,0 +1,632 @@
+// Band-limited angular spectrum propagation (forward and adjoint) for gfx950, fused with
+// the guard-band padding, the amplitude/phase glue, the 2x zero padding, the centre crops
+// and |.|^2.
+//
+// Reference behaviour:
+//   soft_window_pad            model_2d_gaussian_rgb.py:186-241
+//   reconstruct glue           train_2d_gaussian_rgb.py:89-95
+//   propagator / custom()      utils/propagator.py:41-64, 248-290 (odak zero_pad,
+//                              crop_center, BLASM transfer function)
+//
+// Per channel c the forward is three passes over HBM:
+//   F1  rows   : build the padded, tapered, amplitude-clamped field on the fly from the
+//                raster output, place it in the centre of a 2x row, FFT along the row;
+//                only the pH non-zero rows of the 2x grid are transformed (X [pH, nv]).
+//   F2  columns: load the pH non-zero entries of CG adjacent columns, FFT along the
+//                column, keep the spectrum in registers, and for every plane d multiply
+//                by H_d (FP64 phase evaluated on the fly, no cached transfer function),
+//                inverse FFT along the column and store only the rows that survive the
+//                final crop (Y_d [oh, nv]).
+//   F3  rows   : inverse FFT along the row, store only the cropped columns: the complex
+//                field at the plane (kept for the adjoint) and its intensity.
+// The adjoint runs the transposed passes in reverse (A3 rows, A2 columns with the
+// conjugate transfer functions summed over planes, A1 rows + the glue adjoint), so the
+// full 2x spectrum never goes to HBM in either direction.
+#include "common.h"
+#include "cvqpg.h"
+#include "fft.h"
+
+namespace cvqpg {
+
+constexpr int MAXD = 16;
+constexpr int ROW_NT = 512;
+constexpr int ROW_LCAP = 8704;   // longest row transform (complex elements)
+constexpr int COL_NT = 1024;
+constexpr int COL_LCAP = 6144;   // column length x columns per workgroup
+constexpr int COL_PER = COL_LCAP / COL_NT;
+
+struct Optics {
+  int C, D, H, W, pH, pW, nu, nv;
+  int pad_h, pad_w;      // hologram offset inside the padded field
+  int zr0, zc0;          // offset of the padded field on the 2x grid (odak zero_pad)
+  int or0, oc0, oh, ow;  // output crop on the 2x grid
+  int gr0;               // first 2x-grid row of the hologram (adjoint output rows)
+  int cg;                // columns per column workgroup
+  float norm;            // 1 / (nu nv)
+  double fr0, frs;       // centred row-frequency grid  f = fr0 + i frs
+  double fc0, fcs;       // centred column-frequency grid
+  double inv_lam2[3];
+  double z[MAXD];
+  float fxmax[MAXD][3];   // band limit on |FX| (column frequency)
+  float fymax[MAXD][3];   // band limit on |FY| (row frequency)
+  int ap_open;
+  int band_active;         // 0 when no band limit and no evanescence can occur on the grid
+  double ap_r2, apr0, aprs, apc0, apcs;
+};
+
+struct BlasmWS {
+  float2* tw_r;   // [nu]
+  float2* tw_c;   // [nv]
+  float2* X;      // [C][pH][nv]   (forward pass 1 output; adjoint pass 2 output [C][H][nv])
+  float2* Y;      // [D][C][oh][nv] (forward pass 2 output; adjoint pass 3 output)
+  size_t bytes;
+};
+
+static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int oh) {
+  Carver cv(base);
+  BlasmWS w;
+  const size_t nu = 2 * (size_t)pH, nv = 2 * (size_t)pW;
+  w.tw_r = cv.take<float2>(nu);
+  w.tw_c = cv.take<float2>(nv);
+  const size_t xr = (size_t)(pH > H ? pH : H);
+  w.X = cv.take<float2>((size_t)C * xr * nv);
+  w.Y = cv.take<float2>((size_t)D * C * oh * nv);
+  w.bytes = cv.bytes();
+  return w;
+}
+
+__global__ void k_twiddles(float2* __restrict__ tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double s, c;
+  sincospi(2.0 * (double)m / (double)n, &s, &c);
+  tw[m] = make_float2((float)c, (float)-s);
+}
+
+// ---------------------------------------------------------------------------------
+// transfer function
+// ---------------------------------------------------------------------------------
+// Per spectral point the column passes keep only sqrt(1/lambda^2 - FX^2 - FY^2) in FP64
+// (negative: no transmission at any plane); the band-limit test is re-evaluated per
+// plane only when the host found that it can remove anything on this grid.
+__device__ __forceinline__ void tf_freqs(const Optics& o, int kr, int kc, int& ir, int& ic,
+                                         double& fy, double& fx) {
+  // natural FFT index -> centred grid index (the ifftshift of utils/propagator.py:63)
+  ir = kr + (o.nu >> 1) >= o.nu ? kr + (o.nu >> 1) - o.nu : kr + (o.nu >> 1);
+  ic = kc + (o.nv >> 1) >= o.nv ? kc + (o.nv >> 1) - o.nv : kc + (o.nv >> 1);
+  fy = o.fr0 + (double)ir * o.frs;
+  fx = o.fc0 + (double)ic * o.fcs;
+}
+
+__device__ __forceinline__ double tf_point(const Optics& o, int c, int kr, int kc) {
+  int ir, ic;
+  double fy, fx;
+  tf_freqs(o, kr, kc, ir, ic, fy, fx);
+  const double rad = o.inv_lam2[c] - (fx * fx + fy * fy);
+  double sq = rad >= 0.0 ? sqrt(rad) : -1.0;
+  if (!o.ap_open) {
+    const double X = o.apr0 + (double)ir * o.aprs, Y = o.apc0 + (double)ic * o.apcs;
+    if (!(X * X + Y * Y < o.ap_r2)) sq = -1.0;
+  }
+  return sq;
+}
+
+// H_d at a point: exp(+i 2 pi z sqrt(...)) inside the band, 0 outside; the phase is
+// reduced modulo one revolution in FP64 before the FP32 sin/cos
+__device__ __forceinline__ float2 tf_value(const Optics& o, double sq, int c, int d, int kr,
+                                           int kc) {
+  if (sq < 0.0) return make_float2(0.f, 0.f);
+  if (o.band_active) {
+    int ir, ic;
+    double fy, fx;
+    tf_freqs(o, kr, kc, ir, ic, fy, fx);
+    if (!((float)fabs(fx) < o.fxmax[d][c]) || !((float)fabs(fy) < o.fymax[d][c]))
+      return make_float2(0.f, 0.f);
+  }
+  const double rev = o.z[d] * sq;
+  const float fr = (float)(rev - floor(rev));
+  return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
+}
+
+// ---------------------------------------------------------------------------------
+// glue: u = clamp(|p|, 0, 1) e^{i angle(p)}  (== p for |p| <= 1, p/|p| otherwise)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float2 glue(float2 p) {
+  const float a2 = p.x * p.x + p.y * p.y;
+  if (a2 > 1.0f) {
+    const float inv = rsqrtf(a2);
+    return make_float2(p.x * inv, p.y * inv);
+  }
+  return p;
+}
+
+// adjoint of glue for gradient g = dL/dRe u + i dL/dIm u (torch's convention, with the
+// gradient of abs and angle defined as 0 at the origin)
+__device__ __forceinline__ float2 glue_adjoint(float2 h, float2 g) {
+  const float a2 = h.x * h.x + h.y * h.y;
+  if (a2 == 0.0f) return make_float2(0.f, 0.f);
+  if (a2 <= 1.0f) return g;
+  const float a = sqrtf(a2);
+  const float hx = h.x / a, hy = h.y / a;
+  const float q = (g.y * hx - g.x * hy) / a;  // Im(g conj(hh)) / |h|
+  return make_float2(-q * hy, q * hx);         // q * i * hh
+}
+
+__device__ __forceinline__ float taper(int t, int total, int inner_start, int inner_size) {
+  if (t < inner_start)
+    return 0.5f * (1.0f - cosf(3.14159265358979f * (float)(t + 1) / (float)inner_start));
+  const int rs = inner_start + inner_size;
+  if (t >= rs) {
+    const int n_right = total - rs;
+    return 0.5f * (1.0f + cosf(3.14159265358979f * (float)(t - rs + 1) / (float)n_right));
+  }
+  return 1.0f;
+}
+
+// ---------------------------------------------------------------------------------
+// row passes
+// ---------------------------------------------------------------------------------
+// F1: padded/tapered/glued row -> FFT along the 2x row.  grid (pH, C)
+template <int MAXR>
+__global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ field,
+                                                    Optics o, fft::Plan plan,
+                                                    float2* __restrict__ X) {
+  extern __shared__ float2 buf[];
+  const int r = blockIdx.x, c = blockIdx.y;
+  const int sr = min(max(r - o.pad_h, 0), o.H - 1);
+  const float wy = taper(r, o.pH, o.pad_h, o.H);
+  const float2* src = field + ((size_t)c * o.H + sr) * o.W;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
+    const int q = j - o.zc0;
+    float2 u = make_float2(0.f, 0.f);
+    if (q >= 0 && q < o.pW) {
+      const int sc = min(max(q - o.pad_w, 0), o.W - 1);
+      const float w = wy * taper(q, o.pW, o.pad_w, o.W);
+      const float2 h = src[sc];
+      u = glue(make_float2(h.x * w, h.y * w));
+    }
+    buf[j] = u;
+  }
+  __syncthreads();
+  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+}
+
+// F1 for the propagator interface: u = amplitude e^{i phase} on [C, pH, pW].  grid (pH, C)
+template <int MAXR>
+__global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restrict__ phase,
+                                                          const float* __restrict__ amp,
+                                                          Optics o, fft::Plan plan,
+                                                          float2* __restrict__ X) {
+  extern __shared__ float2 buf[];
+  const int r = blockIdx.x, c = blockIdx.y;
+  const size_t base = ((size_t)c * o.pH + r) * o.pW;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
+    const int q = j - o.zc0;
+    float2 u = make_float2(0.f, 0.f);
+    if (q >= 0 && q < o.pW) {
+      const float a = amp[base + q];
+      float s, co;
+      sincosf(phase[base + q], &s, &co);
+      u = make_float2(a * co, a * s);
+    }
+    buf[j] = u;
+  }
+  __syncthreads();
+  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+}
+
+// F3: inverse FFT along the row, crop, field and intensity.  grid (oh, D*C)
+template <int MAXR>
+__global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y, Optics o,
+                                                    fft::Plan plan, float2* __restrict__ out,
+                                                    float* __restrict__ inten) {
+  extern __shared__ float2 buf[];
+  const int r = blockIdx.x, dc = blockIdx.y;
+  const float2* src = Y + ((size_t)dc * o.oh + r) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[j] = src[j];
+  __syncthreads();
+  fft::run<-1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  for (int x = threadIdx.x; x < o.ow; x += ROW_NT) {
+    const float2 v = buf[o.oc0 + x];
+    const float2 f = make_float2(v.x * o.norm, v.y * o.norm);
+    if (out) out[ob + x] = f;
+    inten[ob + x] = f.x * f.x + f.y * f.y;
+  }
+}
+
+// A3: adjoint of F3.  g_out = 2 dL/dI * out (times the 1/(nu nv) of the forward), placed in
+// the cropped columns of a zero 2x row, forward FFT.  grid (oh, D*C)
+template <int MAXR>
+__global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ grad_inten,
+                                                    const float2* __restrict__ out, Optics o,
+                                                    fft::Plan plan, float2* __restrict__ Z) {
+  extern __shared__ float2 buf[];
+  const int r = blockIdx.x, dc = blockIdx.y;
+  const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
+    const int x = j - o.oc0;
+    float2 g = make_float2(0.f, 0.f);
+    if (x >= 0 && x < o.ow) {
+      const float s = 2.0f * grad_inten[ob + x] * o.norm;
+      const float2 f = out[ob + x];
+      g = make_float2(s * f.x, s * f.y);
+    }
+    buf[j] = g;
+  }
+  __syncthreads();
+  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+}
+
+// A1: adjoint of F1 on the hologram rows: unnormalised inverse FFT along the row, the
+// hologram columns, adjoint of the glue.  grid (H, C)
+template <int MAXR>
+__global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G,
+                                                    const float2* __restrict__ field, Optics o,
+                                                    fft::Plan plan,
+                                                    float2* __restrict__ grad_field) {
+  extern __shared__ float2 buf[];
+  const int y = blockIdx.x, c = blockIdx.y;
+  const float2* src = G + ((size_t)c * o.H + y) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[j] = src[j];
+  __syncthreads();
+  fft::run<-1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  const size_t fb = ((size_t)c * o.H + y) * o.W;
+  const int c0 = o.zc0 + o.pad_w;
+  for (int x = threadIdx.x; x < o.W; x += ROW_NT)
+    grad_field[fb + x] = glue_adjoint(field[fb + x], buf[c0 + x]);
+}
+
+// ---------------------------------------------------------------------------------
+// column passes (CG adjacent columns per workgroup, XCD-aware block order)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_block(int hb, int per) { return (hb & 7) * per + (hb >> 3); }
+
+// F2: column FFT, per plane transfer function, inverse column FFT, crop rows.
+// grid (8 * per, C)
+template <int CG, int MAXR>
+__global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X, Optics o,
+                                                    fft::Plan plan_fwd, int per,
+                                                    float2* __restrict__ Y) {
+  extern __shared__ float2 buf[];
+  const int lb = xcd_block(blockIdx.x, per);
+  const int col0 = lb * CG;
+  if (col0 >= o.nv) return;
+  const int c = blockIdx.y;
+  const int n = o.nu;
+  const int total = n * CG;
+  for (int e = threadIdx.x; e < total; e += COL_NT) {
+    const int i = e / CG, g = e % CG;
+    const int r = i - o.zr0;
+    buf[e] = (r >= 0 && r < o.pH) ? X[((size_t)c * o.pH + r) * o.nv + col0 + g]
+                                  : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd);
+  float2 S[COL_PER];
+  double P[COL_PER];
+#pragma unroll
+  for (int b = 0; b < COL_PER; ++b) {
+    const int e = threadIdx.x + b * COL_NT;
+    if (e < total) {
+      S[b] = buf[e];
+      P[b] = tf_point(o, c, e / CG, col0 + e % CG);
+    }
+  }
+  for (int d = 0; d < o.D; ++d) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < COL_PER; ++b) {
+      const int e = threadIdx.x + b * COL_NT;
+      if (e < total) buf[e] = fft::cmul(S[b], tf_value(o, P[b], c, d, e / CG, col0 + e % CG));
+    }
+    __syncthreads();
+    fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd);
+    float2* dst = Y + ((size_t)d * o.C + c) * o.oh * o.nv;
+    for (int e = threadIdx.x; e < o.oh * CG; e += COL_NT) {
+      const int i = e / CG, g = e % CG;
+      dst[(size_t)i * o.nv + col0 + g] = buf[(o.or0 + i) * CG + g];
+    }
+  }
+}
+
+// A2: for every plane, column FFT of the adjoint rows, multiply by conj(H_d), sum; then
+// unnormalised inverse column FFT and keep the hologram rows.  grid (8 * per, C)
+template <int CG, int MAXR>
+__global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z, Optics o,
+                                                    fft::Plan plan, int per,
+                                                    float2* __restrict__ G) {
+  extern __shared__ float2 buf[];
+  const int lb = xcd_block(blockIdx.x, per);
+  const int col0 = lb * CG;
+  if (col0 >= o.nv) return;
+  const int c = blockIdx.y;
+  const int total = o.nu * CG;
+  float2 A[COL_PER];
+  double P[COL_PER];
+#pragma unroll
+  for (int b = 0; b < COL_PER; ++b) {
+    const int e = threadIdx.x + b * COL_NT;
+    A[b] = make_float2(0.f, 0.f);
+    if (e < total) P[b] = tf_point(o, c, e / CG, col0 + e % CG);
+  }
+  for (int d = 0; d < o.D; ++d) {
+    const float2* src = Z + ((size_t)d * o.C + c) * o.oh * o.nv;
+    for (int e = threadIdx.x; e < total; e += COL_NT) {
+      const int i = e / CG, g = e % CG;
+      const int r = i - o.or0;
+      buf[e] = (r >= 0 && r < o.oh) ? src[(size_t)r * o.nv + col0 + g] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan);
+#pragma unroll
+    for (int b = 0; b < COL_PER; ++b) {
+      const int e = threadIdx.x + b * COL_NT;
+      if (e < total) {
+        const float2 h = tf_value(o, P[b], c, d, e / CG, col0 + e % CG);
+        const float2 v = buf[e];
+        A[b].x += v.x * h.x + v.y * h.y;  // v * conj(h)
+        A[b].y += v.y * h.x - v.x * h.y;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < COL_PER; ++b) {
+    const int e = threadIdx.x + b * COL_NT;
+    if (e < total) buf[e] = A[b];
+  }
+  __syncthreads();
+  fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan);
+  float2* dst = G + (size_t)c * o.H * o.nv;
+  for (int e = threadIdx.x; e < o.H * CG; e += COL_NT) {
+    const int i = e / CG, g = e % CG;
+    dst[(size_t)i * o.nv + col0 + g] = buf[(o.gr0 + i) * CG + g];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------------
+// row kernels are instantiated for radices <= 8 (lean) and <= 17
+#define ROW_LAUNCH(K, PLAN, ...)                                  \
+  do {                                                            \
+    if (fft::max_radix(PLAN) <= 8)                                \
+      hipLaunchKernelGGL(K<8>, __VA_ARGS__);                      \
+    else                                                          \
+      hipLaunchKernelGGL(K<17>, __VA_ARGS__);                     \
+  } while (0)
+
+static int pick_cg(int nu, int nv) {
+  for (int cg = 8; cg >= 1; cg >>= 1)
+    if ((int64_t)nu * cg <= COL_LCAP && nv % cg == 0) return cg;
+  return 0;
+}
+
+static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
+                       const double* wavelengths, const double* distances, double pitch,
+                       double aperture_radius, int grid, int full_plane) {
+  CVQPG_CHECK_ARG(C >= 1 && C <= 3, "channels must be 1..3 (got %d)", C);
+  CVQPG_CHECK_ARG(D >= 1 && D <= MAXD, "planes must be 1..%d (got %d)", MAXD, D);
+  CVQPG_CHECK_ARG(H > 0 && W > 0 && pH >= H && pW >= W, "invalid sizes");
+  CVQPG_CHECK_ARG(pH % 2 == 0 && pW % 2 == 0 && H % 2 == 0 && W % 2 == 0,
+                  "image and padded sizes must be even (odak centre crop)");
+  CVQPG_CHECK_ARG(wavelengths && distances, "wavelengths/distances are NULL");
+  CVQPG_CHECK_ARG(pitch > 0, "pixel pitch must be > 0");
+  CVQPG_CHECK_ARG(grid == 0 || grid == 1, "grid must be 0 (odak) or 1 (fft)");
+  o.C = C; o.D = D; o.H = H; o.W = W; o.pH = pH; o.pW = pW;
+  o.nu = 2 * pH; o.nv = 2 * pW;
+  CVQPG_CHECK_ARG(o.nv <= ROW_LCAP, "padded width %d exceeds the supported row length", pW);
+  o.pad_h = (pH - H) / 2;
+  o.pad_w = (pW - W) / 2;
+  o.zr0 = pH - pH / 2;   // odak zero_pad: (2 pH)//2 - pH//2
+  o.zc0 = pW - pW / 2;
+  if (full_plane) {      // crop_center(2x -> padded): start n//4
+    o.or0 = o.nu / 4; o.oc0 = o.nv / 4; o.oh = pH; o.ow = pW;
+  } else {               // then crop_center(size = (H, W)) inside the padded plane
+    o.or0 = o.nu / 4 + pH / 2 - H / 2; o.oc0 = o.nv / 4 + pW / 2 - W / 2; o.oh = H; o.ow = W;
+  }
+  o.gr0 = o.zr0 + o.pad_h;
+  o.cg = pick_cg(o.nu, o.nv);
+  CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length", pH);
+  o.norm = (float)(1.0 / ((double)o.nu * (double)o.nv));
+  const double X = pitch * o.nu, Y = pitch * o.nv;
+  if (grid == 0) {  // odak: linspace(-1/(2dx) + 0.5/(2X), 1/(2dx) - 0.5/(2X), n)
+    o.fr0 = -1.0 / (2 * pitch) + 0.5 / (2 * X);
+    o.frs = (1.0 / pitch - 0.5 / X) / (o.nu - 1);
+    o.fc0 = -1.0 / (2 * pitch) + 0.5 / (2 * Y);
+    o.fcs = (1.0 / pitch - 0.5 / Y) / (o.nv - 1);
+  } else {          // exact FFT bins (i - n/2) / X
+    o.fr0 = -(double)(o.nu / 2) / X;
+    o.frs = 1.0 / X;
+    o.fc0 = -(double)(o.nv / 2) / Y;
+    o.fcs = 1.0 / Y;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double lam = c < C ? wavelengths[c] : 1.0;
+    o.inv_lam2[c] = 1.0 / (lam * lam);
+  }
+  for (int d = 0; d < MAXD; ++d)
+    for (int c = 0; c < 3; ++c) {
+      if (d < D && c < C) {
+        const double z = distances[d], lam = wavelengths[c];
+        o.fxmax[d][c] = (float)(1.0 / sqrt((2 * z / X) * (2 * z / X) + 1) / lam);
+        o.fymax[d][c] = (float)(1.0 / sqrt((2 * z / Y) * (2 * z / Y) + 1) / lam);
+      } else {
+        o.fxmax[d][c] = o.fymax[d][c] = 0.0f;
+      }
+    }
+  for (int d = 0; d < MAXD; ++d) o.z[d] = d < D ? distances[d] : 0.0;
+  {
+    const double fymaxabs = fmax(fabs(o.fr0), fabs(o.fr0 + (o.nu - 1) * o.frs));
+    const double fxmaxabs = fmax(fabs(o.fc0), fabs(o.fc0 + (o.nv - 1) * o.fcs));
+    o.band_active = 0;
+    for (int c = 0; c < C; ++c) {
+      if (o.inv_lam2[c] - (fxmaxabs * fxmaxabs + fymaxabs * fymaxabs) < 0) o.band_active = 1;
+      for (int d = 0; d < D; ++d)
+        if (!((float)fxmaxabs < o.fxmax[d][c]) || !((float)fymaxabs < o.fymax[d][c]))
+          o.band_active = 1;
+    }
+  }
+  // circular aperture on the centred grid: linspace(-n/2, n/2, n) per axis
+  const double rmax = sqrt((o.nu / 2.0) * (o.nu / 2.0) + (o.nv / 2.0) * (o.nv / 2.0));
+  o.ap_open = !(aperture_radius > 0) || aperture_radius > rmax + 1.0;
+  o.ap_r2 = aperture_radius * aperture_radius;
+  o.apr0 = -o.nu / 2.0;
+  o.aprs = (double)o.nu / (o.nu - 1);
+  o.apc0 = -o.nv / 2.0;
+  o.apcs = (double)o.nv / (o.nv - 1);
+  return 0;
+}
+
+template <int CG>
+static void launch_f2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  const int nb = o.nv / CG, per = ceil_div(nb, 8);
+  const dim3 g(8 * per, o.C), b(COL_NT);
+  const size_t sh = (size_t)o.nu * CG * sizeof(float2);
+  const int mr = fft::max_radix(pr);
+  if (mr <= 8)
+    hipLaunchKernelGGL((k_f2_cols<CG, 8>), g, b, sh, s, w.X, o, pr, per, w.Y);
+  else if (mr <= 13)
+    hipLaunchKernelGGL((k_f2_cols<CG, 13>), g, b, sh, s, w.X, o, pr, per, w.Y);
+  else
+    hipLaunchKernelGGL((k_f2_cols<CG, 17>), g, b, sh, s, w.X, o, pr, per, w.Y);
+}
+
+template <int CG>
+static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  const int nb = o.nv / CG, per = ceil_div(nb, 8);
+  const dim3 g(8 * per, o.C), b(COL_NT);
+  const size_t sh = (size_t)o.nu * CG * sizeof(float2);
+  const int mr = fft::max_radix(pr);
+  if (mr <= 8)
+    hipLaunchKernelGGL((k_a2_cols<CG, 8>), g, b, sh, s, w.Y, o, pr, per, w.X);
+  else if (mr <= 13)
+    hipLaunchKernelGGL((k_a2_cols<CG, 13>), g, b, sh, s, w.Y, o, pr, per, w.X);
+  else
+    hipLaunchKernelGGL((k_a2_cols<CG, 17>), g, b, sh, s, w.Y, o, pr, per, w.X);
+}
+
+static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc,
+                 hipStream_t s) {
+  CVQPG_CHECK_ARG(fft::make_plan(o.nu, pr), "FFT length %d has a prime factor above 17", o.nu);
+  CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "FFT length %d has a prime factor above 17", o.nv);
+  pr.tw = w.tw_r;
+  pc.tw = w.tw_c;
+  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(o.nu, 256)), dim3(256), 0, s, w.tw_r, o.nu);
+  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(o.nv, 256)), dim3(256), 0, s, w.tw_c, o.nv);
+  return 0;
+}
+
+static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan& pr,
+                             const fft::Plan& pc, float* out_field, float* intensity,
+                             hipStream_t s) {
+  switch (o.cg) {
+    case 8: launch_f2<8>(w, o, pr, s); break;
+    case 4: launch_f2<4>(w, o, pr, s); break;
+    case 2: launch_f2<2>(w, o, pr, s); break;
+    default: launch_f2<1>(w, o, pr, s); break;
+  }
+  ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), dim3(ROW_NT),
+                     (size_t)o.nv * sizeof(float2), s, w.Y, o, pc,
+                     reinterpret_cast<float2*>(out_field), intensity);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace cvqpg
+
+using namespace cvqpg;
+
+extern "C" size_t cvqpg_blasm_workspace_size(int channels, int planes, int height, int width,
+                                             int pad_height, int pad_width, int full_plane) {
+  const int oh = full_plane ? pad_height : height;
+  (void)width;
+  return carve_blasm(nullptr, channels, planes, height, pad_height, pad_width, oh).bytes;
+}
+
+extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height, int width,
+                                   int pad_height, int pad_width, int planes,
+                                   const double* wavelengths, const double* distances,
+                                   double pixel_pitch, double aperture_radius, int grid,
+                                   void* ws, size_t ws_bytes, float* out_field, float* intensity,
+                                   void* stream) {
+  Optics o;
+  if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
+                           wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
+    return rc;
+  CVQPG_CHECK_ARG(field && intensity && ws, "NULL field/intensity/workspace");
+  const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  fft::Plan pr, pc;
+  if (int rc = plans(o, w, pr, pc, s)) return rc;
+  ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), dim3(ROW_NT), (size_t)o.nv * sizeof(float2), s,
+                     reinterpret_cast<const float2*>(field), o, pc, w.X);
+  CVQPG_LAUNCH_CHECK();
+  return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
+}
+
+extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intensity,
+                                    const float* out_field, int channels, int height, int width,
+                                    int pad_height, int pad_width, int planes,
+                                    const double* wavelengths, const double* distances,
+                                    double pixel_pitch, double aperture_radius, int grid,
+                                    void* ws, size_t ws_bytes, float* grad_field, void* stream) {
+  Optics o;
+  if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
+                           wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
+    return rc;
+  CVQPG_CHECK_ARG(field && grad_intensity && out_field && grad_field && ws,
+                  "NULL argument pointer");
+  const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  fft::Plan pr, pc;
+  if (int rc = plans(o, w, pr, pc, s)) return rc;
+  ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), dim3(ROW_NT),
+                     (size_t)o.nv * sizeof(float2), s, grad_intensity,
+                     reinterpret_cast<const float2*>(out_field), o, pc, w.Y);
+  switch (o.cg) {
+    case 8: launch_a2<8>(w, o, pr, s); break;
+    case 4: launch_a2<4>(w, o, pr, s); break;
+    case 2: launch_a2<2>(w, o, pr, s); break;
+    default: launch_a2<1>(w, o, pr, s); break;
+  }
+  ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), dim3(ROW_NT), (size_t)o.nv * sizeof(float2), s,
+                     w.X, reinterpret_cast<const float2*>(field), o, pc,
+                     reinterpret_cast<float2*>(grad_field));
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitude, int channels,
+                                       int pad_height, int pad_width, int planes,
+                                       const double* wavelengths, const double* distances,
+                                       double pixel_pitch, double aperture_radius, int grid,
+                                       void* ws, size_t ws_bytes, float* intensity,
+                                       void* stream) {
+  Optics o;
+  if (int rc = make_optics(o, channels, pad_height, pad_width, pad_height, pad_width, planes,
+                           wavelengths, distances, pixel_pitch, aperture_radius, grid, 1))
+    return rc;
+  CVQPG_CHECK_ARG(phase && amplitude && intensity && ws, "NULL argument pointer");
+  const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  fft::Plan pr, pc;
+  if (int rc = plans(o, w, pr, pc, s)) return rc;
+  ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), dim3(ROW_NT),
+                     (size_t)o.nv * sizeof(float2), s, phase, amplitude, o, pc, w.X);
+  CVQPG_LAUNCH_CHECK();
+  return forward_cols_rows(w, o, pr, pc, nullptr, intensity, s);
+}

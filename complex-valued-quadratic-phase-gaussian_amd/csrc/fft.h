@@ -1,0 +1,323 @@
+// Mixed-radix Stockham FFT that runs inside one workgroup on data held in LDS.
+//
+// A sequence of length n (or CG interleaved sequences: element i of sequence g is at
+// buf[i * CG + g]) is transformed in place.  Each radix-R stage with
+// ns = product of the earlier radices does, for butterfly j in [0, n/R):
+//     k = j mod ns
+//     v[r] = x[j + r n/R] * W_{ns R}^{r k}            (r = 0..R-1)
+//     v    = DFT_R(v)
+//     y[(j / ns) ns R + k + r ns] = v[r]
+// which leaves the result in natural order (Stockham autosort).  All of a thread's
+// inputs for a stage are read into registers before the barrier and written after it,
+// so a single LDS buffer suffices.
+//
+// Twiddles come from a table W_n^m = exp(-2 pi i m / n), m in [0, n), evaluated in FP64
+// and rounded to FP32 once.  DIR = +1 is the forward transform (e^{-i}), DIR = -1 the
+// unnormalised inverse (e^{+i}).  Radices 2, 3, 4, 5 and 8 have dedicated butterflies;
+// the odd primes 7, 11, 13 and 17 use a symmetric direct DFT with compile-time roots.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cvqpg {
+namespace fft {
+
+constexpr int MAX_STAGES = 16;
+
+struct Plan {
+  int n;
+  int nstages;
+  int radix[MAX_STAGES];
+  const float2* tw;  // [n]
+};
+
+// Factorisation into supported radices; powers of two become radix-8 and radix-4
+// stages (never radix 2 unless n has a single factor 2); returns false if n has a
+// prime factor above 17.
+inline bool make_plan(int n, Plan& p) {
+  p.n = n;
+  p.nstages = 0;
+  if (n < 2) return false;
+  int m = n, a = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++a;
+  }
+  int n8 = a / 3, n4 = 0, n2 = 0;
+  if (a % 3 == 1) {
+    if (n8 > 0) {
+      --n8;
+      n4 = 2;
+    } else {
+      n2 = 1;
+    }
+  } else if (a % 3 == 2) {
+    n4 = 1;
+  }
+  for (int i = 0; i < n8; ++i) p.radix[p.nstages++] = 8;
+  for (int i = 0; i < n4; ++i) p.radix[p.nstages++] = 4;
+  for (int i = 0; i < n2; ++i) p.radix[p.nstages++] = 2;
+  const int odd[] = {3, 5, 7, 11, 13, 17};
+  for (int r : odd) {
+    while (m % r == 0) {
+      if (p.nstages >= MAX_STAGES) return false;
+      p.radix[p.nstages++] = r;
+      m /= r;
+    }
+  }
+  return m == 1;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// multiply by -i (DIR=+1) or +i (DIR=-1)
+template <int DIR>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return DIR > 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int DIR>
+__device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int m) {
+  const float2 w = tw[m];
+  return DIR > 0 ? w : make_float2(w.x, -w.y);
+}
+
+template <int DIR>
+__device__ __forceinline__ void bfly2(float2& a, float2& b) {
+  const float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2* v) {
+  const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  const float2 b0 = cadd(v[1], v[3]), b1 = rot90<DIR>(csub(v[1], v[3]));
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, b1);
+  v[3] = csub(a1, b1);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* v) {
+  constexpr float S = 0.70710678118654752f;
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // o[k] *= W_8^k
+  o[1] = DIR > 0 ? make_float2(S * (o[1].x + o[1].y), S * (o[1].y - o[1].x))
+                 : make_float2(S * (o[1].x - o[1].y), S * (o[1].y + o[1].x));
+  o[2] = rot90<DIR>(o[2]);
+  o[3] = DIR > 0 ? make_float2(S * (-o[3].x + o[3].y), S * (-o[3].y - o[3].x))
+                 : make_float2(S * (-o[3].x - o[3].y), S * (-o[3].y + o[3].x));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft3(float2* v) {
+  constexpr float C1 = -0.5f;
+  constexpr float S1 = 0.86602540378443865f;  // sin(2 pi / 3)
+  const float2 s = cadd(v[1], v[2]);
+  const float2 d = csub(v[1], v[2]);
+  const float2 m = make_float2(v[0].x + C1 * s.x, v[0].y + C1 * s.y);
+  // forward: v1 = m - i S1 d, v2 = m + i S1 d
+  const float2 t = DIR > 0 ? make_float2(S1 * d.y, -S1 * d.x) : make_float2(-S1 * d.y, S1 * d.x);
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, t);
+  v[2] = csub(m, t);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft5(float2* v) {
+  constexpr float C1 = 0.30901699437494742f;   // cos(2 pi / 5)
+  constexpr float C2 = -0.80901699437494742f;  // cos(4 pi / 5)
+  constexpr float S1 = 0.95105651629515357f;   // sin(2 pi / 5)
+  constexpr float S2 = 0.58778525229247313f;   // sin(4 pi / 5)
+  const float2 s1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+  const float2 s2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+  const float2 a1 = make_float2(v[0].x + C1 * s1.x + C2 * s2.x, v[0].y + C1 * s1.y + C2 * s2.y);
+  const float2 a2 = make_float2(v[0].x + C2 * s1.x + C1 * s2.x, v[0].y + C2 * s1.y + C1 * s2.y);
+  // b = S1 d1 + S2 d2 ; c = S2 d1 - S1 d2 ; forward multiplies them by -i
+  const float2 b = make_float2(S1 * d1.x + S2 * d2.x, S1 * d1.y + S2 * d2.y);
+  const float2 c = make_float2(S2 * d1.x - S1 * d2.x, S2 * d1.y - S1 * d2.y);
+  const float2 bi = rot90<DIR>(b), ci = rot90<DIR>(c);
+  v[0] = cadd(v[0], cadd(s1, s2));
+  v[1] = cadd(a1, bi);
+  v[4] = csub(a1, bi);
+  v[2] = cadd(a2, ci);
+  v[3] = csub(a2, ci);
+}
+
+// Compile-time R-th roots of unity (Taylor series in double after reduction to [-pi, pi]),
+// so the direct DFT of the odd radices uses literal constants instead of registers.
+constexpr double kPi = 3.14159265358979323846;
+constexpr double cx_sin(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 40; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 40; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+template <int R>
+struct Roots {
+  float c[R];
+  float s[R];
+  constexpr Roots() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      double a = 2.0 * kPi * (double)m / (double)R;
+      if (a > kPi) a -= 2.0 * kPi;
+      c[m] = (float)cx_cos(a);
+      s[m] = (float)cx_sin(a);
+    }
+  }
+};
+
+// direct DFT of odd length R in the symmetric form
+//   X_k, X_{R-k} = v0 + sum_j s_j cos(2 pi j k / R)  -/+ i sum_j d_j sin(2 pi j k / R)
+// with s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j} (forward; the inverse swaps the signs)
+template <int R, int DIR>
+__device__ __forceinline__ void dft_odd(float2* v) {
+  constexpr int h = (R - 1) / 2;
+  constexpr Roots<R> rt{};
+  float2 sj[h], dj[h];
+#pragma unroll
+  for (int j = 1; j <= h; ++j) {
+    sj[j - 1] = cadd(v[j], v[R - j]);
+    dj[j - 1] = csub(v[j], v[R - j]);
+  }
+  const float2 v0 = v[0];
+  float2 tot = v0;
+#pragma unroll
+  for (int j = 0; j < h; ++j) tot = cadd(tot, sj[j]);
+#pragma unroll
+  for (int k = 1; k <= h; ++k) {
+    float2 a = v0, b = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 1; j <= h; ++j) {
+      const float c = rt.c[(j * k) % R], sn = rt.s[(j * k) % R];
+      a.x = fmaf(sj[j - 1].x, c, a.x);
+      a.y = fmaf(sj[j - 1].y, c, a.y);
+      b.x = fmaf(dj[j - 1].x, sn, b.x);
+      b.y = fmaf(dj[j - 1].y, sn, b.y);
+    }
+    // forward: X_k = a - i b ; X_{R-k} = a + i b
+    const float2 ib = make_float2(-b.y, b.x);
+    v[k] = DIR > 0 ? csub(a, ib) : cadd(a, ib);
+    v[R - k] = DIR > 0 ? cadd(a, ib) : csub(a, ib);
+  }
+  v[0] = tot;
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 2) {
+    bfly2<DIR>(v[0], v[1]);
+  } else if constexpr (R == 3) {
+    dft3<DIR>(v);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v);
+  } else if constexpr (R == 5) {
+    dft5<DIR>(v);
+  } else if constexpr (R == 8) {
+    dft8<DIR>(v);
+  } else {
+    dft_odd<R, DIR>(v);
+  }
+}
+
+// One radix-R stage on CG interleaved sequences of length n held in buf.
+// LCAP is the LDS capacity in complex elements (n * CG <= LCAP); NT threads.
+template <int R, int DIR, int CG, int NT, int LCAP>
+__device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* __restrict__ tw) {
+  constexpr int MAXB = (LCAP / R + NT - 1) / NT;
+  const int cnt = n / R;
+  const int total = cnt * CG;
+  const int tstep = n / (ns * R);
+  float2 v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int w = threadIdx.x + b * NT;
+    dst[b] = -1;
+    if (w < total) {
+      const int col = w % CG;
+      const int j = w / CG;
+      const int k = j % ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * cnt) * CG + col];
+      if (ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twid<DIR>(tw, r * k * tstep));
+      }
+      dft<R, DIR>(v[b]);
+      dst[b] = ((j / ns) * ns * R + k) * CG + col;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    if (dst[b] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[dst[b] + r * ns * CG] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform; the caller has written the input to buf and synchronised.  MAXR bounds
+// the radices compiled into this instance: kernels whose plans only use radices <= 8
+// instantiate MAXR = 8 and do not pay the registers of the large direct-DFT stages.
+template <int DIR, int CG, int NT, int LCAP, int MAXR = 17>
+__device__ void run(float2* buf, const Plan& p) {
+  int ns = 1;
+  for (int s = 0; s < p.nstages; ++s) {
+    const int R = p.radix[s];
+    switch (R) {
+      case 2: stage<2, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 3: stage<3, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 4: stage<4, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 5: stage<5, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 7: stage<7, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 8: stage<8, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 11:
+        if constexpr (MAXR >= 11) stage<11, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        break;
+      case 13:
+        if constexpr (MAXR >= 13) stage<13, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        break;
+      case 17:
+        if constexpr (MAXR >= 17) stage<17, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        break;
+      default: break;
+    }
+    ns *= R;
+  }
+}
+
+inline int max_radix(const Plan& p) {
+  int m = 0;
+  for (int s = 0; s < p.nstages; ++s) m = p.radix[s] > m ? p.radix[s] : m;
+  return m;
+}
+
+}  // namespace fft
+}  // namespace cvqpg

@@ -721,30 +721,44 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   return 0;
 }
 
-extern "C" int cvqpg_render_forward(const float* means, const float* inv_cov,
-                                    const float* colours, const float* phase,
-                                    const float* opacities, const float* curvature,
-                                    const float* area, const float* k, int n, int channels,
-                                    float dxy2, int width, int height, void* ws, size_t ws_bytes,
-                                    int64_t capacity, const int32_t* point_list,
-                                    const int32_t* ranges, float* field, void* stream) {
+template <int C>
+static void launch_pack(const float* means, const float* inv_cov, const float* colours,
+                        const float* phase, const float* opac, const float* curv,
+                        const float* area, const float* k, float dxy2, int n, float* packed,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_pack<C>, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const float2*>(means), inv_cov, colours, phase, opac, curv,
+                     area, k, dxy2, n, packed);
+}
+
+extern "C" int cvqpg_raster_pack(const float* means, const float* inv_cov, const float* colours,
+                                 const float* phase, const float* opacities,
+                                 const float* curvature, const float* area, const float* k, int n,
+                                 int channels, float dxy2, int width, int height, void* ws,
+                                 size_t ws_bytes, int64_t capacity, void* stream) {
   if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
-  CVQPG_CHECK_ARG(field && ranges && point_list && k, "NULL output/ranges/point_list/k");
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(means && inv_cov && colours && phase && opacities && curvature && area && k,
+                  "NULL Gaussian parameter pointer");
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  if (channels == 3)
+    launch_pack<3>(means, inv_cov, colours, phase, opacities, curvature, area, k, dxy2, n, w.packed, s);
+  else
+    launch_pack<1>(means, inv_cov, colours, phase, opacities, curvature, area, k, dxy2, n, w.packed, s);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_render_forward_packed(int n, int channels, int width, int height, void* ws,
+                                           size_t ws_bytes, int64_t capacity,
+                                           const int32_t* point_list, const int32_t* ranges,
+                                           float* field, void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  CVQPG_CHECK_ARG(field && ranges && point_list, "NULL field/ranges/point_list");
   hipStream_t s = (hipStream_t)stream;
   RasterWS w = carve(ws, n, capacity);
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
-  if (n > 0) {
-    CVQPG_CHECK_ARG(means && inv_cov && colours && phase && opacities && curvature && area,
-                    "NULL Gaussian parameter pointer");
-    if (channels == 3)
-      hipLaunchKernelGGL(k_pack<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                         reinterpret_cast<const float2*>(means), inv_cov, colours, phase,
-                         opacities, curvature, area, k, dxy2, n, w.packed);
-    else
-      hipLaunchKernelGGL(k_pack<1>, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                         reinterpret_cast<const float2*>(means), inv_cov, colours, phase,
-                         opacities, curvature, area, k, dxy2, n, w.packed);
-  }
   if (channels == 3)
     hipLaunchKernelGGL(k_render_fwd<3>, dim3(gx * gy), dim3(256), 0, s,
                        reinterpret_cast<const int2*>(ranges), point_list,
@@ -759,6 +773,74 @@ extern "C" int cvqpg_render_forward(const float* means, const float* inv_cov,
   return 0;
 }
 
+extern "C" int cvqpg_render_backward_tiles(const float* grad_field, const float* k, int n,
+                                           int channels, int width, int height, void* ws,
+                                           size_t ws_bytes, int64_t capacity,
+                                           const int32_t* point_list, const int32_t* ranges,
+                                           void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(grad_field && ranges && point_list && k, "NULL grad/ranges/point_list/k");
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_render_bwd<3>, dim3(gx * gy), dim3(256), 0, s,
+                       reinterpret_cast<const int2*>(ranges), point_list,
+                       reinterpret_cast<const float4*>(w.packed), w.rect,
+                       reinterpret_cast<const float2*>(grad_field), k, w.misc, width, height, gx,
+                       reinterpret_cast<float4*>(w.partials));
+  else
+    hipLaunchKernelGGL(k_render_bwd<1>, dim3(gx * gy), dim3(256), 0, s,
+                       reinterpret_cast<const int2*>(ranges), point_list,
+                       reinterpret_cast<const float4*>(w.packed), w.rect,
+                       reinterpret_cast<const float2*>(grad_field), k, w.misc, width, height, gx,
+                       reinterpret_cast<float4*>(w.partials));
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_render_backward_reduce(const float* colours, const float* curvature,
+                                            const float* area, int n, int channels, float dxy2,
+                                            int width, int height, void* ws, size_t ws_bytes,
+                                            int64_t capacity, float* d_means, float* d_inv_cov,
+                                            float* d_colours, float* d_phase, float* d_opacities,
+                                            float* d_curvature, float* d_area, void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(colours && curvature && area, "NULL colours/curvature/area");
+  CVQPG_CHECK_ARG(d_means && d_inv_cov && d_colours && d_phase && d_opacities && d_curvature &&
+                      d_area,
+                  "NULL gradient output pointer");
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  const dim3 gb(ceil_div(n, 256)), tb(256);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_bwd_reduce<3>, gb, tb, 0, s, reinterpret_cast<const float4*>(w.partials),
+                       w.rect, w.counts, w.misc, colours, curvature, area, dxy2, n, d_means,
+                       d_inv_cov, d_colours, d_phase, d_opacities, d_curvature, d_area);
+  else
+    hipLaunchKernelGGL(k_bwd_reduce<1>, gb, tb, 0, s, reinterpret_cast<const float4*>(w.partials),
+                       w.rect, w.counts, w.misc, colours, curvature, area, dxy2, n, d_means,
+                       d_inv_cov, d_colours, d_phase, d_opacities, d_curvature, d_area);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_render_forward(const float* means, const float* inv_cov,
+                                    const float* colours, const float* phase,
+                                    const float* opacities, const float* curvature,
+                                    const float* area, const float* k, int n, int channels,
+                                    float dxy2, int width, int height, void* ws, size_t ws_bytes,
+                                    int64_t capacity, const int32_t* point_list,
+                                    const int32_t* ranges, float* field, void* stream) {
+  if (int rc = cvqpg_raster_pack(means, inv_cov, colours, phase, opacities, curvature, area, k, n,
+                                 channels, dxy2, width, height, ws, ws_bytes, capacity, stream))
+    return rc;
+  return cvqpg_render_forward_packed(n, channels, width, height, ws, ws_bytes, capacity,
+                                     point_list, ranges, field, stream);
+}
+
 extern "C" int cvqpg_render_backward(const float* grad_field, const float* means,
                                      const float* inv_cov, const float* colours,
                                      const float* phase, const float* opacities,
@@ -769,43 +851,15 @@ extern "C" int cvqpg_render_backward(const float* grad_field, const float* means
                                      float* d_means, float* d_inv_cov, float* d_colours,
                                      float* d_phase, float* d_opacities, float* d_curvature,
                                      float* d_area, void* stream) {
-  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
-  if (n == 0) return 0;
-  CVQPG_CHECK_ARG(grad_field && ranges && point_list && k, "NULL grad/ranges/point_list/k");
-  CVQPG_CHECK_ARG(means && inv_cov && colours && phase && opacities && curvature && area,
-                  "NULL Gaussian parameter pointer");
-  CVQPG_CHECK_ARG(d_means && d_inv_cov && d_colours && d_phase && d_opacities && d_curvature &&
-                      d_area,
-                  "NULL gradient output pointer");
-  hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity);
-  const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
-  const dim3 gb(ceil_div(n, 256)), tb(256);
-  if (channels == 3) {
-    hipLaunchKernelGGL(k_pack<3>, gb, tb, 0, s, reinterpret_cast<const float2*>(means), inv_cov,
-                       colours, phase, opacities, curvature, area, k, dxy2, n, w.packed);
-    hipLaunchKernelGGL(k_render_bwd<3>, dim3(gx * gy), dim3(256), 0, s,
-                       reinterpret_cast<const int2*>(ranges), point_list,
-                       reinterpret_cast<const float4*>(w.packed), w.rect,
-                       reinterpret_cast<const float2*>(grad_field), k, w.misc, width, height, gx,
-                       reinterpret_cast<float4*>(w.partials));
-    hipLaunchKernelGGL(k_bwd_reduce<3>, gb, tb, 0, s, reinterpret_cast<const float4*>(w.partials),
-                       w.rect, w.counts, w.misc, colours, curvature, area, dxy2, n, d_means,
-                       d_inv_cov, d_colours, d_phase, d_opacities, d_curvature, d_area);
-  } else {
-    hipLaunchKernelGGL(k_pack<1>, gb, tb, 0, s, reinterpret_cast<const float2*>(means), inv_cov,
-                       colours, phase, opacities, curvature, area, k, dxy2, n, w.packed);
-    hipLaunchKernelGGL(k_render_bwd<1>, dim3(gx * gy), dim3(256), 0, s,
-                       reinterpret_cast<const int2*>(ranges), point_list,
-                       reinterpret_cast<const float4*>(w.packed), w.rect,
-                       reinterpret_cast<const float2*>(grad_field), k, w.misc, width, height, gx,
-                       reinterpret_cast<float4*>(w.partials));
-    hipLaunchKernelGGL(k_bwd_reduce<1>, gb, tb, 0, s, reinterpret_cast<const float4*>(w.partials),
-                       w.rect, w.counts, w.misc, colours, curvature, area, dxy2, n, d_means,
-                       d_inv_cov, d_colours, d_phase, d_opacities, d_curvature, d_area);
-  }
-  CVQPG_LAUNCH_CHECK();
-  return 0;
+  if (int rc = cvqpg_raster_pack(means, inv_cov, colours, phase, opacities, curvature, area, k, n,
+                                 channels, dxy2, width, height, ws, ws_bytes, capacity, stream))
+    return rc;
+  if (int rc = cvqpg_render_backward_tiles(grad_field, k, n, channels, width, height, ws,
+                                           ws_bytes, capacity, point_list, ranges, stream))
+    return rc;
+  return cvqpg_render_backward_reduce(colours, curvature, area, n, channels, dxy2, width, height,
+                                      ws, ws_bytes, capacity, d_means, d_inv_cov, d_colours,
+                                      d_phase, d_opacities, d_curvature, d_area, stream);
 }
 
 extern "C" int cvqpg_qpf_render_forward(const float* means, const float* inv_cov,

@@ -77,19 +77,60 @@ def bin_gaussians(means_2d, radii, width, height, channels=3):
 
 def render_forward(means_2d, inv_cov, colours, phase, opacities, curvature, area, k, dxy2,
                    width, height, bins):
+    L = _lib.lib()
     n, C = colours.shape
     field = torch.empty(C, height, width, 2, dtype=torch.float32, device=means_2d.device)
-    _lib.check(_lib.lib().cvqpg_render_forward(
+    st = _lib.stream_handle(means_2d.device)
+    ws, nb, cap = _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity
+    _lib.check(L.cvqpg_raster_pack(
         _lib.ptr(means_2d), _lib.ptr(inv_cov), _lib.ptr(colours), _lib.ptr(phase),
         _lib.ptr(opacities), _lib.ptr(curvature), _lib.ptr(area), _lib.ptr(k), n, C, float(dxy2),
-        width, height, _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity,
-        _lib.ptr(bins.point_list), _lib.ptr(bins.ranges), _lib.ptr(field),
-        _lib.stream_handle(means_2d.device)))
+        width, height, ws, nb, cap, st))
+    timer.wrap("k_render_fwd", lambda: _lib.check(L.cvqpg_render_forward_packed(
+        n, C, width, height, ws, nb, cap, _lib.ptr(bins.point_list), _lib.ptr(bins.ranges),
+        _lib.ptr(field), st)))
     return torch.view_as_complex(field)
+
+
+class StageTimer:
+    """Optional HIP-event timing of single raster kernels on the launch stream.
+
+    When ``enabled``, the adjoint tile kernel (k_render_bwd) and the forward tile
+    kernel (k_render_fwd) are bracketed by events on the stream they are launched on;
+    ``mean_ms(name)`` returns their average duration."""
+
+    def __init__(self):
+        self.enabled = False
+        self.events = {}
+
+    def wrap(self, name, fn):
+        if not self.enabled:
+            return fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        out = fn()
+        b.record()
+        self.events.setdefault(name, []).append((a, b))
+        return out
+
+    def mean_ms(self, name):
+        ev = self.events.get(name, [])
+        if not ev:
+            return None
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+
+    def reset(self):
+        self.events = {}
+
+
+timer = StageTimer()
 
 
 def render_backward(grad_field, means_2d, inv_cov, colours, phase, opacities, curvature, area, k,
                     dxy2, width, height, bins):
+    L = _lib.lib()
     n, C = colours.shape
     dev = means_2d.device
     g = torch.view_as_real(grad_field.resolve_conj().contiguous()).contiguous()
@@ -98,12 +139,18 @@ def render_backward(grad_field, means_2d, inv_cov, colours, phase, opacities, cu
             torch.empty(n, device=dev), torch.empty(n, device=dev), torch.empty(n, device=dev)]
     if n == 0:
         return [o.zero_() for o in outs]
-    _lib.check(_lib.lib().cvqpg_render_backward(
-        _lib.ptr(g), _lib.ptr(means_2d), _lib.ptr(inv_cov), _lib.ptr(colours), _lib.ptr(phase),
+    st = _lib.stream_handle(dev)
+    ws, nb, cap = _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity
+    _lib.check(L.cvqpg_raster_pack(
+        _lib.ptr(means_2d), _lib.ptr(inv_cov), _lib.ptr(colours), _lib.ptr(phase),
         _lib.ptr(opacities), _lib.ptr(curvature), _lib.ptr(area), _lib.ptr(k), n, C, float(dxy2),
-        width, height, _lib.ptr(bins.ws), bins.ws.numel(), bins.capacity,
-        _lib.ptr(bins.point_list), _lib.ptr(bins.ranges), *[_lib.ptr(o) for o in outs],
-        _lib.stream_handle(dev)))
+        width, height, ws, nb, cap, st))
+    timer.wrap("k_render_bwd", lambda: _lib.check(L.cvqpg_render_backward_tiles(
+        _lib.ptr(g), _lib.ptr(k), n, C, width, height, ws, nb, cap, _lib.ptr(bins.point_list),
+        _lib.ptr(bins.ranges), st)))
+    _lib.check(L.cvqpg_render_backward_reduce(
+        _lib.ptr(colours), _lib.ptr(curvature), _lib.ptr(area), n, C, float(dxy2), width, height,
+        ws, nb, cap, *[_lib.ptr(o) for o in outs], st))
     return outs
 
 

@@ -93,6 +93,33 @@ int cvqpg_render_backward(const float *grad_field, const float *means, const flo
                           float *d_colours, float *d_phase, float *d_opacities,
                           float *d_curvature, float *d_area, void *stream);
 
+/* Stages of the two calls above, for callers that reuse the packed parameters
+ * between forward and adjoint (the fused training step) or time one kernel:
+ *   pack            per-Gaussian derived constants into the workspace
+ *                   (the per-pair-invariant part of cuda/forward.cu:98-127);
+ *   forward_packed  renderTileKernel<C> on packed parameters (cuda/forward.cu:62-162);
+ *   backward_tiles  the per-tile adjoint kernel writing one partial record per key
+ *                   (cuda/backward.cu:22-148 without its atomics, :150-165);
+ *   backward_reduce the per-Gaussian sum of the records and the final gradients. */
+int cvqpg_raster_pack(const float *means, const float *inv_cov, const float *colours,
+                      const float *phase, const float *opacities, const float *curvature,
+                      const float *area, const float *k, int n, int channels, float dxy2,
+                      int width, int height, void *ws, size_t ws_bytes, int64_t capacity,
+                      void *stream);
+int cvqpg_render_forward_packed(int n, int channels, int width, int height, void *ws,
+                                size_t ws_bytes, int64_t capacity, const int32_t *point_list,
+                                const int32_t *ranges, float *field, void *stream);
+int cvqpg_render_backward_tiles(const float *grad_field, const float *k, int n, int channels,
+                                int width, int height, void *ws, size_t ws_bytes,
+                                int64_t capacity, const int32_t *point_list,
+                                const int32_t *ranges, void *stream);
+int cvqpg_render_backward_reduce(const float *colours, const float *curvature, const float *area,
+                                 int n, int channels, float dxy2, int width, int height, void *ws,
+                                 size_t ws_bytes, int64_t capacity, float *d_means,
+                                 float *d_inv_cov, float *d_colours, float *d_phase,
+                                 float *d_opacities, float *d_curvature, float *d_area,
+                                 void *stream);
+
 /* cvqpg_bin_count + cvqpg_bin_sort + cvqpg_render_forward in one stream-ordered
  * call with no host read-back (graph-capturable); *num_pairs receives P.
  * Replaces qpf_render_forward (cuda/raster.cu:30-105). */
@@ -103,6 +130,54 @@ int cvqpg_qpf_render_forward(const float *means, const float *inv_cov, const flo
                              size_t ws_bytes, int64_t capacity, float *field,
                              int32_t *point_list, int32_t *ranges, int32_t *num_pairs,
                              void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Band-limited angular spectrum propagation.                                */
+/* The reference has no native code here: these replace the PyTorch/odak    */
+/* path soft_window_pad (model_2d_gaussian_rgb.py:186-241) -> reconstruct    */
+/* glue (train_2d_gaussian_rgb.py:89-95) -> propagator.reconstruct           */
+/* (utils/propagator.py:224-290, custom() :41-64) -> centre crop             */
+/* (train_2d_gaussian_rgb.py:193).                                           */
+/*                                                                           */
+/* Common parameters: `channels` C (1..3) and `planes` D (1..16); the image  */
+/* is height x width, the padded field pad_height x pad_width (both even),   */
+/* the FFT grid (2 pad_height) x (2 pad_width) (prime factors <= 17, 2       */
+/* pad_width <= 8704, 2 pad_height <= 6144).  `wavelengths` [C] and          */
+/* `distances` [D] are HOST arrays of doubles (metres); `aperture_radius` is */
+/* the circular Fourier aperture radius in samples of the 2x grid (<= 0 or   */
+/* larger than the grid: open); `grid` 0 = odak's half-sample frequency grid,*/
+/* 1 = exact FFT bins.  The transfer-function phase is evaluated in FP64.    */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes; full_plane = 1 for cvqpg_blasm_reconstruct, 0 otherwise. */
+size_t cvqpg_blasm_workspace_size(int channels, int planes, int height, int width, int pad_height,
+                                  int pad_width, int full_plane);
+
+/* field [C,H,W] complex64 (raster output) -> padded, tapered, amplitude-clamped,
+ * propagated to every plane and cropped to H x W: out_field [D,C,H,W] complex64
+ * (may be NULL if no backward follows) and intensity [D,C,H,W] float32. */
+int cvqpg_blasm_forward(const float *field, int channels, int height, int width, int pad_height,
+                        int pad_width, int planes, const double *wavelengths,
+                        const double *distances, double pixel_pitch, double aperture_radius,
+                        int grid, void *ws, size_t ws_bytes, float *out_field, float *intensity,
+                        void *stream);
+
+/* Adjoint of cvqpg_blasm_forward: grad_intensity [D,C,H,W] -> grad_field [C,H,W]
+ * complex64 (dL/dRe + i dL/dIm of the raster field), through the crops, the
+ * conjugate transfer functions, the zero padding and the amplitude/phase glue. */
+int cvqpg_blasm_backward(const float *field, const float *grad_intensity, const float *out_field,
+                         int channels, int height, int width, int pad_height, int pad_width,
+                         int planes, const double *wavelengths, const double *distances,
+                         double pixel_pitch, double aperture_radius, int grid, void *ws,
+                         size_t ws_bytes, float *grad_field, void *stream);
+
+/* propagator.reconstruct(phase, amplitude) for one frame per channel
+ * (utils/propagator.py:276-290): phase, amplitude [C,pH,pW] float32 ->
+ * intensity [D,C,pH,pW] float32 over the whole padded plane. */
+int cvqpg_blasm_reconstruct(const float *phase, const float *amplitude, int channels,
+                            int pad_height, int pad_width, int planes, const double *wavelengths,
+                            const double *distances, double pixel_pitch, double aperture_radius,
+                            int grid, void *ws, size_t ws_bytes, float *intensity, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,140 @@
+"""GPU parity of the fused pad -> glue -> BLASM -> crop kernels against the FP64 oracle.
+
+Bars (relative L2, FP32 kernels vs FP64 oracle with an FP64 transfer-function phase):
+  * per-plane intensities <= 1e-4 (oracle.model.soft_window_pad -> oracle.model.glue ->
+    oracle.optics.BLASM -> crop, i.e. model_2d_gaussian_rgb.py:186-241,
+    train_2d_gaussian_rgb.py:89-95,193, utils/propagator.py:248-290);
+  * gradient w.r.t. the raster field <= 1e-4 against torch autograd of the same FP64
+    pipeline;
+  * at full 1080p size, the z = 0 plane equals |glue(field)|^2 (a size-independent
+    identity of the transform pair).
+FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import model as omodel
+from oracle import optics as ooptics
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def make_field(C, H, W, seed, peak=1.3):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H, 0:W]
+    f = np.zeros((C, H, W), np.complex128)
+    for _ in range(12):
+        cx, cy, s = rng.uniform(0, W), rng.uniform(0, H), rng.uniform(2, 8)
+        a = rng.uniform(0.2, 1.0, C) * np.exp(1j * rng.uniform(0, 2 * np.pi, C))
+        g = np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2 * s * s))
+        f += a[:, None, None] * g[None]
+    f *= peak / np.abs(f).max()
+    f += 0.05 * (rng.normal(size=f.shape) + 1j * rng.normal(size=f.shape))
+    return f.astype(np.complex64)
+
+
+def oracle_planes(field, pad, wl, pitch, dists, aperture, grid, requires_grad=False):
+    h = torch.as_tensor(field).to(torch.complex128)
+    if requires_grad:
+        h.requires_grad_()
+    padded = omodel.soft_window_pad(h, pad)
+    u, _, _ = omodel.glue(padded)
+    prop = ooptics.BLASM(pad, wl, pitch, dists, aperture, grid)
+    inten = prop.reconstruct(u)
+    H, W = field.shape[1:]
+    return ooptics.crop_center(inten, (H, W)), h
+
+
+def rel(a, b):
+    a = torch.as_tensor(a).cpu().to(torch.float64 if not torch.as_tensor(a).is_complex() else torch.complex128)
+    b = torch.as_tensor(b).cpu().to(a.dtype)
+    return float((a - b).abs().norm() / b.abs().norm())
+
+
+CASES = [  # (C, H, W, pH, pW, distances, grid)
+    (3, 40, 48, 112, 130, [0.0, 2.5e-3, 5e-3], "odak"),      # 224 = 2^5 7, 260 = 2^2 5 13
+    (1, 64, 64, 352, 320, [1e-3, 4e-3], "odak"),             # 704 = 2^6 11, 640 = 2^7 5
+    (3, 32, 32, 320, 320, [0.0, 6e-3], "fft"),               # the default guard band for 32x32
+    (1, 56, 40, 144, 96, [3e-3], "odak"),                     # 288 = 2^5 9, 192 = 2^6 3
+]
+
+
+@pytest.mark.parametrize("C,H,W,pH,pW,dists,grid", CASES)
+def test_forward_and_adjoint_match_fp64(pkg, C, H, W, pH, pW, dists, grid):
+    from cvqpg_amd.optics import HologramOptics, propagate
+    wl = [639e-9, 515e-9, 473e-9][:C] if C == 3 else [515e-9]
+    pitch = 8e-6
+    ap = 10 * max(W, H) * 10  # open aperture, as the training script configures it
+    field = make_field(C, H, W, seed=pH + pW)
+    ref, h = oracle_planes(field, (pH, pW), wl, pitch, dists, ap, grid, requires_grad=True)
+    opt = HologramOptics((W, H), (pH, pW), wl, pitch, dists, aperture_size=ap, grid=grid)
+    f_dev = torch.as_tensor(field).to(DEV).requires_grad_()
+    inten = propagate(f_dev, opt)
+    torch.cuda.synchronize()
+    errs = [rel(inten[d].detach(), ref[d].detach()) for d in range(len(dists))]
+    print(f"case C={C} {H}x{W} pad {pH}x{pW} grid={grid}: plane rel L2 "
+          + " ".join(f"{e:.1e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-4
+    w = torch.as_tensor(np.random.default_rng(1).uniform(0.5, 1.5, ref.shape))
+    (ref * w).sum().backward()
+    (inten * w.to(DEV, torch.float32)).sum().backward()
+    eg = rel(f_dev.grad, h.grad)
+    print(f"  gradient rel L2 {eg:.1e}", flush=True)
+    assert eg <= 1e-4
+
+
+def test_aperture_and_band_limit_active(pkg):
+    """A finite Fourier aperture (export-time setting) and a grid small enough for the
+    band limit to bite."""
+    from cvqpg_amd.optics import HologramOptics
+    C, H, W, pH, pW = 1, 32, 32, 64, 64
+    wl, pitch, dists = [515e-9], 8e-6, [2e-2]
+    for ap in (40.0, 1e6):
+        field = make_field(C, H, W, seed=3, peak=0.9)
+        ref, _ = oracle_planes(field, (pH, pW), wl, pitch, dists, ap, "odak")
+        opt = HologramOptics((W, H), (pH, pW), wl, pitch, dists, aperture_size=ap)
+        inten, _ = opt.forward(torch.as_tensor(field).to(DEV))
+        e = rel(inten, ref)
+        print(f"aperture {ap}: rel L2 {e:.1e}", flush=True)
+        assert e <= 1e-4
+
+
+def test_reconstruct_full_plane(pkg):
+    """propagator.reconstruct interface: phase/amplitude on the padded plane."""
+    from cvqpg_amd.optics import HologramOptics
+    C, pH, pW = 3, 96, 128
+    wl, pitch, dists = [639e-9, 515e-9, 473e-9], 8e-6, [1e-3, 3e-3]
+    rng = np.random.default_rng(7)
+    ph = rng.uniform(0, 2 * np.pi, (C, pH, pW)).astype(np.float32)
+    am = rng.uniform(0, 1, (C, pH, pW)).astype(np.float32)
+    prop = ooptics.BLASM((pH, pW), wl, pitch, dists, 1e6, "odak")
+    u = torch.polar(torch.as_tensor(am, dtype=torch.float64), torch.as_tensor(ph, dtype=torch.float64))
+    ref = prop.reconstruct(u)
+    opt = HologramOptics((pW, pH), (pH, pW), wl, pitch, dists, aperture_size=1e6)
+    got = opt.reconstruct_planes(torch.as_tensor(ph).to(DEV), torch.as_tensor(am).to(DEV))
+    e = rel(got, ref)
+    print(f"reconstruct rel L2 {e:.1e}", flush=True)
+    assert e <= 1e-4
+
+
+def test_full_size_z0_identity(pkg):
+    """1080p RGB with the training guard band: the z = 0 plane is |glue(field)|^2."""
+    from cvqpg_amd.optics import HologramOptics
+    C, H, W = 3, 1080, 1920
+    pH, pW = 1408, 2240
+    rng = np.random.default_rng(0)
+    f = (rng.normal(size=(C, H, W)) + 1j * rng.normal(size=(C, H, W))) * 0.5
+    field = torch.as_tensor(f.astype(np.complex64)).to(DEV)
+    opt = HologramOptics((W, H), (pH, pW), [639e-9, 515e-9, 473e-9], 8e-6, [0.0, 2e-3],
+                         aperture_size=10 * W)
+    inten, _ = opt.forward(field, keep_field=False)
+    a = field.abs()
+    expect = torch.where(a > 1, torch.ones_like(a), a) ** 2
+    e = float((inten[0] - expect).norm() / expect.norm())
+    print(f"full-size z=0 identity rel L2 {e:.1e}", flush=True)
+    assert e <= 1e-5
+    assert torch.isfinite(inten).all()
