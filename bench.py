@@ -29,8 +29,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", default="auto", choices=["auto", "autograd"],
-                    help="auto: fastest available step; autograd: reference-structured step")
+    ap.add_argument("--path", default="fused", choices=["fused", "autograd"],
+                    help="fused: all-HIP graph-captured step; autograd: reference-structured "
+                         "step (HIP raster, PyTorch elsewhere)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default=WORKLOAD)
     return ap.parse_args()
@@ -92,6 +93,16 @@ def cpu_baseline(spec, seed=0):
                        f"elementwise work not counted; {t_r:.1f}s + {t_fft:.1f}s measured")}
 
 
+def stage_report(trainer, steps=3):
+    """Eager per-stage GPU times (HIP events between the stages), for diagnostics."""
+    import torch
+    ev = {}
+    for _ in range(steps):
+        trainer.step_timed(ev)
+    torch.cuda.synchronize()
+    return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in ev.items()}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -101,49 +112,73 @@ def main():
     import torch
     sys.path.insert(0, ROOT)
     import __graft_entry__ as ge
-    pkg = ge.load_pkg()
-    from cvqpg_amd import rasterizer, train, synthetic
+    ge.load_pkg()
+    from cvqpg_amd import fused, rasterizer, synthetic, train
 
     dev = torch.device("cuda:0")
     spec = synthetic.WORKLOADS[args.workload]
     W, H, N = spec["W"], spec["H"], spec["N"]
     C = len(spec["wavelengths"])
+    print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     wl = train.Workload(args.workload, device=dev, seed=0)
-    trainer = train.AutogradTrainer(wl, backend="torch")
-    path = "autograd: HIP raster + torch.fft BLASM + torch loss/Adan"
-    for _ in range(args.warmup):
-        trainer.step()
-    torch.cuda.synchronize()
-    rasterizer.timer.enabled = True
-    rasterizer.timer.reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        trainer.step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    rasterizer.timer.enabled = False
-    ms = 1e3 * dt / args.steps
-    P = rasterizer._capacity_hint.get((N, W, H), 0)
-    pairs = P * 256
+    pH, pW = synthetic.pad_size_for(W, H)
     f_fwd, f_bwd = raster_flops_per_pair(C)
-    k_ms = rasterizer.timer.mean_ms("k_render_bwd")
-    k_fwd_ms = rasterizer.timer.mean_ms("k_render_fwd")
+    if args.path == "autograd":
+        trainer = train.AutogradTrainer(wl, backend="torch")
+        path = "autograd: HIP raster + torch.fft BLASM + torch loss/Adan"
+        for _ in range(args.warmup):
+            trainer.step()
+        torch.cuda.synchronize()
+        rasterizer.timer.enabled = True
+        rasterizer.timer.reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            trainer.step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rasterizer.timer.enabled = False
+        P = rasterizer._capacity_hint.get((N, W, H), 0)
+        k_ms = rasterizer.timer.mean_ms("k_render_bwd")
+        stages = {"k_render_fwd": rasterizer.timer.mean_ms("k_render_fwd"), "k_render_bwd": k_ms}
+    else:
+        trainer = fused.FusedTrainer(wl, warmup_iters=400, seed=0)
+        path = "fused HIP step, hipGraph replay (prologue, bin, sort, raster, BLASM, loss, adjoints, Adan)"
+        trainer.step()
+        stages = stage_report(trainer)
+        print(f"[bench] eager stage ms: {json.dumps(stages)}", file=sys.stderr, flush=True)
+        trainer.capture(split="render_bwd")
+        for _ in range(args.warmup):
+            trainer.run()
+        torch.cuda.synchronize()
+        ev = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            trainer.run(split_events=ev)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+        if not trainer.check_capacity():
+            raise RuntimeError("key capacity exceeded during the timed steps")
+        P = trainer.last_pairs
+    ms = 1e3 * dt / args.steps
+    pairs = P * 256
     achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
-    roof = {"kernel": "k_render_bwd (raster adjoint)", "bound": "mfma", "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None, "avg_ms": k_ms, "pairs": pairs, "flop_per_pair": f_bwd,
-            "k_render_fwd_avg_ms": k_fwd_ms}
+    roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)", "bound": "mfma",
+            "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": k_ms,
+            "pairs": pairs, "flop_per_pair": f_bwd}
     cpu = None
     if not args.no_cpu_baseline:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(spec)
-    pH, pW = synthetic.pad_size_for(W, H)
     out = {"metric": METRIC, "value": 1e3 / ms, "unit": "step/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
            "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
-                      "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path},
+                      "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
+                      "stage_ms_eager": stages},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

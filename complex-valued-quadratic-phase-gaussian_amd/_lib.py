@@ -47,6 +47,16 @@ SIGNATURES = {
                                                                     _p, _c_size, _p, _p]),
     "cvqpg_blasm_reconstruct": (_c_int, [_p, _p] + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64, _c_int,
                                                                    _p, _c_size, _p, _p]),
+    "cvqpg_loss_workspace_size": (_c_size, [_c_int] * 4),
+    "cvqpg_loss_forward_backward": (_c_int, [_p, _p, _p, _p, _p, ctypes.c_uint64, _p, _c_int,
+                                             _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),
+    "cvqpg_prologue_forward": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _c_f32, _c_f32,
+                                        _c_int, _c_f32, _p, _c_size, _c_i64, _p, _p, _p]),
+    "cvqpg_raster_backward_raw": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_f32, _c_f32,
+                                           _c_int, _c_f32, _p, _c_int, _p, _c_size, _c_i64, _p,
+                                           _p, _p]),
+    "cvqpg_adan_step": (_c_int, [_p, _p, _p, _p, _p, _c_int, _p, _c_f32, _c_f32, _c_int, _c_f32,
+                                 _c_f32, _c_f32, _c_f32, _c_f32, _c_int, _c_int, _p, _p]),
 }
 
 _LIB = None
@@ -89,6 +99,21 @@ def doubles(values):
     """Host array of doubles for the C ABI's small parameter lists."""
     vals = [float(v) for v in values]
     return (ctypes.c_double * len(vals))(*vals)
+
+
+def ptr_array(tensors):
+    """Host array of device pointers (e.g. the seven raw parameter tensors)."""
+    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+
+def int64s(values):
+    vals = [int(v) for v in values]
+    return (ctypes.c_int64 * len(vals))(*vals)
+
+
+def floats(values):
+    vals = [float(v) for v in values]
+    return (ctypes.c_float * len(vals))(*vals)
 
 
 def stream_handle(device=None):

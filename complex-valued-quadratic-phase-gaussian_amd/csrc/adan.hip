@@ -1,0 +1,141 @@
+// Fused optimiser tail for gfx950: nan_to_num (done where the gradients are written),
+// global-norm clip and the Adan update of all seven parameter tensors in one launch.
+//
+// Reference behaviour:
+//   nan_to_num + clip_grad_norm_(1.0)   train_2d_gaussian_rgb.py:205-209
+//   Adan.step / _single_tensor_adan     utils/optimizer.py:105-262 (betas (0.98, 0.92,
+//                                       0.99), eps 1e-8, weight decay 0, no_prox False,
+//                                       its own max_grad_norm 0 -> factor 1)
+//   learning-rate groups                train_2d_gaussian_rgb.py:73-86
+//   MeansCosineScheduler                train_2d_gaussian_rgb.py:58-70
+//   curvature warm-up (frozen, no grad) train_2d_gaussian_rgb.py:150-160, 181-185
+//
+// The iteration index lives in device memory so a captured HIP graph of the whole step
+// replays correctly: the learning rate of the means group, the bias corrections and the
+// curvature warm-up are all derived from it inside the kernel, and k_advance bumps it
+// at the end of the step.
+#include "common.h"
+#include "cvqpg.h"
+
+namespace cvqpg {
+
+struct AdanParams {
+  float* param[7];
+  int64_t off[8];       // segment offsets into the flat gradient / state arrays
+  float lr[7];          // per-segment learning rate (segment 0 uses the cosine schedule)
+  float lr0, eta_min;   // means schedule
+  int num_itrs;
+  float b1, b2, b3, eps;
+  float max_norm;
+  int curv_seg;         // segment index of the curvature tensor
+  int curv_warmup;      // iterations with the curvature frozen
+  int curv_enabled;     // 0: curvature never trained (flat primitive)
+};
+
+__global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restrict__ grads,
+                                              float* __restrict__ state,
+                                              const float* __restrict__ sumsq, int n_partials,
+                                              const int32_t* __restrict__ iteration) {
+  __shared__ double red[256];
+  __shared__ float s_coef;
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int i = t; i < n_partials; i += 256) acc += (double)sumsq[i];
+  red[t] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double norm = sqrt(red[0]);
+    const double coef = (double)A.max_norm / (norm + 1e-6);
+    s_coef = (float)(coef < 1.0 ? coef : 1.0);
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + t;
+  const int64_t total = A.off[7];
+  if (i >= total) return;
+  int seg = 0;
+#pragma unroll
+  for (int k = 1; k < 7; ++k) seg += i >= A.off[k] ? 1 : 0;
+  const int itr = iteration[0];
+  const int step = itr + 1;  // every group's step counter advances on every call
+  bool first = step == 1;
+  if (seg == A.curv_seg) {
+    if (!A.curv_enabled || itr < A.curv_warmup) return;  // no gradient: untouched
+    first = first || itr == A.curv_warmup;               // state created at unfreeze
+  }
+  float lr = A.lr[seg];
+  if (seg == 0) {
+    const double T = A.num_itrs > 1 ? A.num_itrs : 1;
+    lr = (float)(A.eta_min + (A.lr0 - A.eta_min) * (1.0 + cos(3.141592653589793 * itr / T)) / 2.0);
+  }
+  const float bc1 = 1.0f - powf(A.b1, (float)step);
+  const float bc2 = 1.0f - powf(A.b2, (float)step);
+  const float bc3 = 1.0f - powf(A.b3, (float)step);
+  float* m = state;
+  float* v = state + total;
+  float* dd = state + 2 * total;
+  float* npg = state + 3 * total;
+  const float g = grads[i] * s_coef;
+  const float prev = first ? -g : npg[i];
+  const float diff = prev + g;
+  const float mi = m[i] * A.b1 + (1.0f - A.b1) * g;
+  const float di = dd[i] * A.b2 + (1.0f - A.b2) * diff;
+  const float u = diff * A.b2 + g;
+  const float vi = v[i] * A.b3 + (1.0f - A.b3) * u * u;
+  const float denom = sqrtf(vi) / sqrtf(bc3) + A.eps;
+  float* p = A.param[seg] + (i - A.off[seg]);
+  float pv = *p;
+  pv -= (lr / bc1) * mi / denom;
+  pv -= (lr * A.b2 / bc2) * di / denom;
+  *p = pv;
+  m[i] = mi;
+  v[i] = vi;
+  dd[i] = di;
+  npg[i] = -g;
+}
+
+__global__ void k_advance(int32_t* iteration) { iteration[0] += 1; }
+
+}  // namespace cvqpg
+
+using namespace cvqpg;
+
+extern "C" int cvqpg_adan_step(float* const* params, const int64_t* sizes, const float* grads,
+                               float* state, const float* sumsq_partials, int n_partials,
+                               const float* group_lr, float lr0, float eta_min, int num_itrs,
+                               float beta1, float beta2, float beta3, float eps, float max_norm,
+                               int curv_warmup, int curv_enabled, int32_t* iteration,
+                               void* stream) {
+  CVQPG_CHECK_ARG(params && sizes && grads && state && sumsq_partials && group_lr && iteration,
+                  "NULL argument pointer");
+  AdanParams A;
+  A.off[0] = 0;
+  for (int k = 0; k < 7; ++k) {
+    CVQPG_CHECK_ARG(params[k] != nullptr && sizes[k] >= 0, "parameter %d invalid", k);
+    A.param[k] = params[k];
+    A.off[k + 1] = A.off[k] + sizes[k];
+    A.lr[k] = group_lr[k];
+  }
+  A.lr0 = lr0;
+  A.eta_min = eta_min;
+  A.num_itrs = num_itrs;
+  A.b1 = beta1;
+  A.b2 = beta2;
+  A.b3 = beta3;
+  A.eps = eps;
+  A.max_norm = max_norm;
+  A.curv_seg = 6;
+  A.curv_warmup = curv_warmup;
+  A.curv_enabled = curv_enabled;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t total = A.off[7];
+  if (total > 0)
+    hipLaunchKernelGGL(k_adan, dim3(ceil_div(total, 256)), dim3(256), 0, s, A, grads, state,
+                       sumsq_partials, n_partials, iteration);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, iteration);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}

@@ -623,6 +623,227 @@ __global__ void k_bwd_reduce(const float4* __restrict__ partials, const int4* __
 }
 
 // ------------------------------------------------------------------------------
+// fused prologue (activations -> covariance -> inverse -> radius -> packed record) and
+// the adjoint reduction chained back to the seven raw parameters
+//   model_2d_gaussian_rgb.py:83-153 (activations, covariance, inverse) and :428-436
+//   (inv_cov stack, area, 5-sigma radius)
+// ------------------------------------------------------------------------------
+constexpr int NPARAM = 7;  // means_2d, pre_act_scales, pre_act_rotation, colours,
+                           // pre_act_phase, pre_act_opacities, curvature
+struct RawPtrs {
+  const float* p[NPARAM];
+};
+
+struct Prologue {
+  int W, H;
+  float dxy2;
+  float curv_lambda;   // wavelength of the curvature bound (min over channels; first for gray)
+  float curv_nyquist;  // scale-aware bound fraction (0.9)
+  int curv_mode;       // 0 off, 1 constant (200 tanh), 2 scale-aware
+};
+
+template <int C>
+struct Act {
+  float tmx, tmy, mx, my, sx, sy, rot, cs, sn, o, tk, cmax, curv;
+  float c00, c01, c11, det, i00, i01, i11, area;
+  bool clamped;
+  float col[C], ph[C];
+};
+
+template <int C>
+__device__ __forceinline__ void activate(const RawPtrs& R, int g, const Prologue& pr, Act<C>& a) {
+  const float2 m = reinterpret_cast<const float2*>(R.p[0])[g];
+  a.tmx = tanhf(m.x);
+  a.tmy = tanhf(m.y);
+  a.mx = (a.tmx + 1.0f) * 0.5f * (float)pr.W;
+  a.my = (a.tmy + 1.0f) * 0.5f * (float)pr.H;
+  const float2 s = reinterpret_cast<const float2*>(R.p[1])[g];
+  a.sx = expf(s.x) + 0.1f;
+  a.sy = expf(s.y) + 0.1f;
+  a.rot = R.p[2][g];
+  constexpr float TWO_PI = 6.283185307179586f;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    a.col[c] = R.p[3][C * g + c];
+    const float x = R.p[4][C * g + c];
+    float r = fmodf(x, TWO_PI);  // Python-style remainder (sign of the divisor)
+    if (r < 0.f) r += TWO_PI;
+    if (r >= TWO_PI) r -= TWO_PI;
+    a.ph[c] = r;
+  }
+  a.o = 1.0f / (1.0f + expf(-R.p[5][g]));
+  const float sig = fmaxf(a.sx, a.sy);
+  a.tk = tanhf(R.p[6][g]);
+  if (pr.curv_mode == 2) {
+    a.cmax = pr.curv_nyquist * pr.curv_lambda / (6.0f * sig * pr.dxy2 + 1e-20f);
+    a.curv = a.cmax * a.tk;
+  } else if (pr.curv_mode == 1) {
+    a.cmax = 200.0f;
+    a.curv = 200.0f * a.tk;
+  } else {
+    a.cmax = 0.f;
+    a.curv = 0.f;
+  }
+  a.cs = cosf(a.rot);
+  a.sn = sinf(a.rot);
+  const float sx2 = a.sx * a.sx, sy2 = a.sy * a.sy;
+  a.c00 = sx2 * a.cs * a.cs + sy2 * a.sn * a.sn + 0.1f;
+  a.c01 = (sx2 - sy2) * a.cs * a.sn;
+  a.c11 = sx2 * a.sn * a.sn + sy2 * a.cs * a.cs + 0.1f;
+  const float det = a.c00 * a.c11 - a.c01 * a.c01;
+  a.clamped = !(det >= 1e-10f);
+  a.det = a.clamped ? 1e-10f : det;
+  const float inv_det = 1.0f / a.det;
+  a.i00 = a.c11 * inv_det;
+  a.i01 = -a.c01 * inv_det;
+  a.i11 = a.c00 * inv_det;
+  a.area = fabsf(a.sx * a.sy);
+}
+
+template <int C>
+__global__ void k_prologue(RawPtrs R, Prologue pr, const float* __restrict__ kvec, int n,
+                           float* __restrict__ means_out, int32_t* __restrict__ radii_out,
+                           float* __restrict__ packed) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Act<C> a;
+  activate<C>(R, g, pr, a);
+  reinterpret_cast<float2*>(means_out)[g] = make_float2(a.mx, a.my);
+  const float mid = 0.5f * (a.c00 + a.c11);
+  const float disc = sqrtf(((a.c00 - a.c11) * 0.5f) * ((a.c00 - a.c11) * 0.5f) + a.c01 * a.c01);
+  radii_out[g] = (int32_t)ceilf(5.0f * sqrtf(mid + disc));
+  const float beta = 0.5f * a.curv * a.area * pr.dxy2;
+  float r[REC];
+  r[0] = a.mx;
+  r[1] = a.my;
+  r[2] = QSCALE * a.i00;
+  r[3] = QSCALE * a.i01;
+  r[4] = QSCALE * a.i11;
+  r[5] = a.o;
+  r[6] = beta;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    r[7 + 3 * c] = a.ph[c] * INV_2PI;
+    r[8 + 3 * c] = kvec[c] * beta * INV_2PI * (1.0f / QSCALE);
+    r[9 + 3 * c] = a.col[c];
+  }
+#pragma unroll
+  for (int i = 7 + 3 * C; i < REC; ++i) r[i] = 0.f;
+  float4* dst = reinterpret_cast<float4*>(packed + (size_t)g * REC);
+#pragma unroll
+  for (int i = 0; i < REC / 4; ++i) dst[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+}
+
+__device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.0f; }
+
+// grads: flat [(7 + 2C) N] in parameter order (means [N,2], scales [N,2], rotation [N],
+// colours [N,C], phase [N,C], opacities [N], curvature [N]); sumsq: one partial per block
+template <int C>
+__global__ __launch_bounds__(256) void k_bwd_reduce_raw(
+    const float4* __restrict__ partials, const int4* __restrict__ rect,
+    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ misc, RawPtrs R,
+    Prologue pr, const int32_t* __restrict__ itr, int curv_warmup, int n,
+    float* __restrict__ grads, float* __restrict__ sumsq) {
+  __shared__ float red[256];
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  float sq = 0.f;
+  if (g < n) {
+    float acc[REC];
+#pragma unroll
+    for (int i = 0; i < REC; ++i) acc[i] = 0.f;
+    if (!misc[1]) {
+      const int e0 = rect[g].x;
+      const int cnt = (int)counts[g];
+      for (int e = e0; e < e0 + cnt; ++e) {
+        const float4* src = partials + (size_t)e * (REC / 4);
+#pragma unroll
+        for (int i = 0; i < REC / 4; ++i) {
+          const float4 v = src[i];
+          acc[4 * i] += v.x;
+          acc[4 * i + 1] += v.y;
+          acc[4 * i + 2] += v.z;
+          acc[4 * i + 3] += v.w;
+        }
+      }
+    }
+    Act<C> a;
+    activate<C>(R, g, pr, a);
+    // gradients with respect to the rasterizer inputs
+    const float Kq = acc[2 * C + 1] * (1.0f / QSCALE);
+    const float d_op = acc[2 * C + 0];
+    const float d_cv = Kq * 0.5f * a.area * pr.dxy2;
+    const float d_ar = Kq * 0.5f * a.curv * pr.dxy2;
+    const float d00 = acc[2 * C + 2], d01 = 2.0f * acc[2 * C + 3], d11 = acc[2 * C + 4];
+    const float dmx = acc[2 * C + 5] * (2.0f / QSCALE), dmy = acc[2 * C + 6] * (2.0f / QSCALE);
+    // inverse <- covariance
+    float gc00, gc01, gc11;
+    if (a.clamped) {
+      const float id = 1.0f / a.det;
+      gc00 = d11 * id;
+      gc11 = d00 * id;
+      gc01 = -d01 * id;
+    } else {
+      const float D = a.det, iD = 1.0f / D, iD2 = iD * iD;
+      gc00 = -d00 * a.c11 * a.c11 * iD2 + d01 * a.c01 * a.c11 * iD2 + d11 * (iD - a.c00 * a.c11 * iD2);
+      gc11 = d00 * (iD - a.c11 * a.c00 * iD2) + d01 * a.c01 * a.c00 * iD2 - d11 * a.c00 * a.c00 * iD2;
+      gc01 = d00 * 2.0f * a.c11 * a.c01 * iD2 + d01 * (-iD - 2.0f * a.c01 * a.c01 * iD2) +
+             d11 * 2.0f * a.c00 * a.c01 * iD2;
+    }
+    // covariance <- scales, rotation
+    const float c2 = a.cs * a.cs, s2 = a.sn * a.sn, csn = a.cs * a.sn;
+    float gsx = 2.0f * a.sx * (gc00 * c2 + gc01 * csn + gc11 * s2);
+    float gsy = 2.0f * a.sy * (gc00 * s2 - gc01 * csn + gc11 * c2);
+    const float grot = (a.sx * a.sx - a.sy * a.sy) *
+                       (-2.0f * csn * gc00 + (c2 - s2) * gc01 + 2.0f * csn * gc11);
+    // area = |sx sy|
+    const float sgn = (a.sx * a.sy) >= 0.f ? 1.0f : -1.0f;
+    gsx += d_ar * sgn * a.sy;
+    gsy += d_ar * sgn * a.sx;
+    // curvature = cmax(sigma) tanh(kappa)
+    float gk = 0.f;
+    if (pr.curv_mode == 2) {
+      const float sig = fmaxf(a.sx, a.sy);
+      gk = d_cv * a.cmax * (1.0f - a.tk * a.tk);
+      const float dcm = -a.cmax * 6.0f * pr.dxy2 / (6.0f * sig * pr.dxy2 + 1e-20f);
+      const float gsig = d_cv * a.tk * dcm;
+      if (a.sx >= a.sy) gsx += gsig; else gsy += gsig;
+    } else if (pr.curv_mode == 1) {
+      gk = d_cv * 200.0f * (1.0f - a.tk * a.tk);
+    }
+    const bool curv_on = pr.curv_mode != 0 && (itr ? itr[0] : 0) >= curv_warmup;
+    // raw parameters
+    const int N = n;
+    float* gm = grads;
+    float* gs = grads + 2 * (size_t)N;
+    float* gr = grads + 4 * (size_t)N;
+    float* gcol = grads + 5 * (size_t)N;
+    float* gph = grads + (5 + C) * (size_t)N;
+    float* gop = grads + (5 + 2 * C) * (size_t)N;
+    float* gcv = grads + (6 + 2 * C) * (size_t)N;
+    float v;
+    v = finite_or_zero(dmx * 0.5f * (float)pr.W * (1.0f - a.tmx * a.tmx)); gm[2 * g] = v; sq += v * v;
+    v = finite_or_zero(dmy * 0.5f * (float)pr.H * (1.0f - a.tmy * a.tmy)); gm[2 * g + 1] = v; sq += v * v;
+    v = finite_or_zero(gsx * (a.sx - 0.1f)); gs[2 * g] = v; sq += v * v;
+    v = finite_or_zero(gsy * (a.sy - 0.1f)); gs[2 * g + 1] = v; sq += v * v;
+    v = finite_or_zero(grot); gr[g] = v; sq += v * v;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      v = finite_or_zero(acc[c]); gcol[C * g + c] = v; sq += v * v;
+      v = finite_or_zero(acc[C + c] * a.col[c]); gph[C * g + c] = v; sq += v * v;
+    }
+    v = finite_or_zero(d_op * a.o * (1.0f - a.o)); gop[g] = v; sq += v * v;
+    v = curv_on ? finite_or_zero(gk) : 0.0f; gcv[g] = v; sq += v * v;
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sumsq[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------
 static int tile_bits(int T) {
@@ -879,4 +1100,73 @@ extern "C" int cvqpg_qpf_render_forward(const float* means, const float* inv_cov
   return cvqpg_render_forward(means, inv_cov, colours, phase, opacities, curvature, area, k, n,
                               channels, dxy2, width, height, ws, ws_bytes, capacity, point_list,
                               ranges, field, stream);
+}
+
+static int make_prologue(Prologue& pr, int width, int height, float dxy2, float curv_lambda,
+                         int curv_mode, float curv_nyquist) {
+  CVQPG_CHECK_ARG(curv_mode >= 0 && curv_mode <= 2, "curv_mode must be 0, 1 or 2");
+  pr.W = width;
+  pr.H = height;
+  pr.dxy2 = dxy2;
+  pr.curv_lambda = curv_lambda;
+  pr.curv_nyquist = curv_nyquist;
+  pr.curv_mode = curv_mode;
+  return 0;
+}
+
+static RawPtrs raw_ptrs(const float* const* raw) {
+  RawPtrs R;
+  for (int i = 0; i < NPARAM; ++i) R.p[i] = raw[i];
+  return R;
+}
+
+extern "C" int cvqpg_prologue_forward(const float* const* raw_params, int n, int channels,
+                                      int width, int height, const float* k, float dxy2,
+                                      float curv_lambda, int curv_mode, float curv_nyquist,
+                                      void* ws, size_t ws_bytes, int64_t capacity,
+                                      float* means_out, int32_t* radii_out, void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(raw_params && k && means_out && radii_out, "NULL argument pointer");
+  for (int i = 0; i < NPARAM; ++i) CVQPG_CHECK_ARG(raw_params[i], "raw parameter %d is NULL", i);
+  Prologue pr;
+  if (int rc = make_prologue(pr, width, height, dxy2, curv_lambda, curv_mode, curv_nyquist)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  const RawPtrs R = raw_ptrs(raw_params);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_prologue<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k, n,
+                       means_out, radii_out, w.packed);
+  else
+    hipLaunchKernelGGL(k_prologue<1>, dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k, n,
+                       means_out, radii_out, w.packed);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_raster_backward_raw(const float* const* raw_params, int n, int channels,
+                                         int width, int height, float dxy2, float curv_lambda,
+                                         int curv_mode, float curv_nyquist,
+                                         const int32_t* iteration, int curv_warmup, void* ws,
+                                         size_t ws_bytes, int64_t capacity, float* grads,
+                                         float* sumsq_partials, void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(raw_params && grads && sumsq_partials, "NULL argument pointer");
+  for (int i = 0; i < NPARAM; ++i) CVQPG_CHECK_ARG(raw_params[i], "raw parameter %d is NULL", i);
+  Prologue pr;
+  if (int rc = make_prologue(pr, width, height, dxy2, curv_lambda, curv_mode, curv_nyquist)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  const RawPtrs R = raw_ptrs(raw_params);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_bwd_reduce_raw<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(w.partials), w.rect, w.counts, w.misc, R,
+                       pr, iteration, curv_warmup, n, grads, sumsq_partials);
+  else
+    hipLaunchKernelGGL(k_bwd_reduce_raw<1>, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(w.partials), w.rect, w.counts, w.misc, R,
+                       pr, iteration, curv_warmup, n, grads, sumsq_partials);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
 }

@@ -1,0 +1,257 @@
+"""All-HIP training step for one scene, graph-capturable.
+
+One iteration of train_2d_gaussian_rgb.py:187-211 as a fixed sequence of libcvqpg calls
+on the current stream:
+
+    prologue      cvqpg_prologue_forward       activations, cov, inverse, radii, packing
+    binning       cvqpg_bin_count, cvqpg_bin_sort
+    raster        cvqpg_render_forward_packed  -> field [C,H,W]
+    optics        cvqpg_blasm_forward          -> plane fields + intensities [D,C,H,W]
+    loss          cvqpg_loss_forward_backward  -> dL/dI, loss
+    optics adj.   cvqpg_blasm_backward         -> dL/dfield
+    raster adj.   cvqpg_render_backward_tiles + cvqpg_raster_backward_raw -> raw grads
+    optimiser     cvqpg_adan_step              clip, Adan, schedule, iteration += 1
+
+Nothing in ``step`` allocates or synchronises, so ``capture`` can record it into a HIP
+graph (torch.cuda.CUDAGraph drives hipStreamBeginCapture/EndCapture on ROCm).  The key
+buffers are sized once from an eager count with headroom; ``check_capacity`` reads the
+device key count back (one host read) so a caller can re-size if the scene grows.
+"""
+import math
+
+import torch
+
+from . import _lib
+from .model import PARAM_NAMES
+from .optics import HologramOptics
+from .rasterizer import workspace_bytes
+
+TILE = 16
+ADAN_BETAS = (0.98, 0.92, 0.99)
+GROUP_LR = {"means_2d": 0.01, "pre_act_scales": 0.005, "pre_act_rotation": 0.001,
+            "colours": 0.0025, "pre_act_phase": 0.0025, "pre_act_opacities": 0.025,
+            "curvature": 0.001}
+
+
+class FusedTrainer:
+    def __init__(self, wl, num_itrs=2001, lr=0.01, warmup_iters=400, primitive="curv",
+                 seed=0, inject_noise=True, headroom=1.25):
+        L = _lib.lib()
+        g = wl.gaussians
+        self.wl = wl
+        self.dev = g.means_2d.device
+        dev = self.dev
+        self.params = [getattr(g, n).data for n in PARAM_NAMES]
+        for p in self.params:
+            assert p.is_contiguous() and p.dtype == torch.float32
+        self.N = g.means_2d.shape[0]
+        self.C = g.channels
+        self.W, self.H = wl.img_size
+        ap = wl.args_prop
+        self.wavelengths = list(ap.wavelengths)
+        self.k = (2 * math.pi / torch.tensor(self.wavelengths, dtype=torch.float64)).float().to(dev)
+        self.dxy2 = float(ap.pixel_pitch) ** 2
+        self.curv_lambda = float(self.wavelengths[0] if self.C == 1 else min(self.wavelengths))
+        self.curv_mode = 0 if primitive == "flat" else 2
+        self.curv_nyquist = float(getattr(ap, "curv_nyquist", 0.9))
+        self.warmup = int(warmup_iters) if primitive != "flat" else 0
+        self.num_itrs, self.lr0 = int(num_itrs), float(lr)
+        self.seed = int(seed)
+        self.T = math.ceil(self.W / TILE) * math.ceil(self.H / TILE)
+        self.optics = HologramOptics(wl.img_size, ap.pad_size, self.wavelengths, ap.pixel_pitch,
+                                     ap.distances, aperture_size=ap.aperture_size)
+        D, C, H, W, N = self.optics.D, self.C, self.H, self.W, self.N
+        self.D = D
+        # per-step buffers
+        self.means = torch.empty(N, 2, device=dev)
+        self.radii = torch.empty(N, dtype=torch.int32, device=dev)
+        self.num_pairs = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.ranges = torch.empty(self.T, 2, dtype=torch.int32, device=dev)
+        self.field = torch.empty(C, H, W, dtype=torch.complex64, device=dev)
+        self.inten = torch.empty(D, C, H, W, device=dev)
+        self.plane_field = torch.empty(D, C, H, W, dtype=torch.complex64, device=dev)
+        self.grad_inten = torch.empty(D, C, H, W, device=dev)
+        self.grad_field = torch.empty(C, H, W, dtype=torch.complex64, device=dev)
+        self.loss = torch.zeros(1, device=dev)
+        nl = L.cvqpg_loss_workspace_size(C, D, H, W)
+        self.loss_ws = torch.empty(int(nl), dtype=torch.uint8, device=dev)
+        self.targets = wl.targets.contiguous().float()
+        self.masks = wl.masks.contiguous().float()
+        if inject_noise:
+            rng = (self.targets.flatten(1).max(1).values - self.targets.flatten(1).min(1).values)
+            self.noise_scale = [1e-3 * float(v) for v in rng.cpu()]
+        else:
+            self.noise_scale = None
+        self.sizes = [p.numel() for p in self.params]
+        self.total = sum(self.sizes)
+        self.grads = torch.empty(self.total, device=dev)
+        self.n_partials = math.ceil(N / 256)
+        self.sumsq = torch.empty(self.n_partials, device=dev)
+        self.state = torch.zeros(4 * self.total, device=dev)
+        self.iteration = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.group_lr = [GROUP_LR[n] for n in PARAM_NAMES]
+        self.graph = None
+        self.capacity = 0
+        self._size_keys(headroom)
+
+    # ------------------------------------------------------------------ sizing
+    def _alloc_raster(self, capacity):
+        self.capacity = int(capacity)
+        nb = workspace_bytes(self.N, self.C, self.W, self.H, self.capacity)
+        self.raster_ws = torch.empty(nb, dtype=torch.uint8, device=self.dev)
+        self.point_list = torch.empty(max(self.capacity, 1), dtype=torch.int32, device=self.dev)
+
+    def _size_keys(self, headroom):
+        """Eager prologue + count with a provisional capacity, then fix the capacity."""
+        self._alloc_raster(max(1024, 16 * self.N))
+        self._prologue()
+        self._count()
+        P = int(self.num_pairs.item())
+        self.last_pairs = P
+        self._alloc_raster(max(1024, int(P * headroom)))
+
+    def check_capacity(self):
+        """Host read of the last key count; True if it still fits the buffers."""
+        P = int(self.num_pairs.item())
+        self.last_pairs = P
+        return P <= self.capacity
+
+    # ------------------------------------------------------------------ stages
+    def _s(self):
+        return _lib.stream_handle(self.dev)
+
+    def _ws(self):
+        return _lib.ptr(self.raster_ws), self.raster_ws.numel(), self.capacity
+
+    def _prologue(self):
+        ws, nb, cap = self._ws()
+        _lib.check(_lib.lib().cvqpg_prologue_forward(
+            _lib.ptr_array(self.params), self.N, self.C, self.W, self.H, _lib.ptr(self.k),
+            self.dxy2, self.curv_lambda, self.curv_mode, self.curv_nyquist, ws, nb, cap,
+            _lib.ptr(self.means), _lib.ptr(self.radii), self._s()))
+
+    def _count(self):
+        ws, nb, cap = self._ws()
+        _lib.check(_lib.lib().cvqpg_bin_count(
+            _lib.ptr(self.means), _lib.ptr(self.radii), self.N, self.W, self.H, ws, nb, cap,
+            _lib.ptr(self.num_pairs), self._s()))
+
+    def _stages(self):
+        L = _lib.lib()
+        S = self._s  # resolved at launch time: the capture stream during capture
+        ws, nb, cap = self._ws()
+        N, C, W, H = self.N, self.C, self.W, self.H
+        o = self.optics
+        ows = o._workspace(self.dev)
+        ns = None if self.noise_scale is None else _lib.floats(self.noise_scale)
+        b1, b2, b3 = ADAN_BETAS
+        fr = lambda t: _lib.ptr(torch.view_as_real(t))
+        return [
+            ("prologue", self._prologue),
+            ("bin_count", self._count),
+            ("bin_sort", lambda: _lib.check(L.cvqpg_bin_sort(
+                _lib.ptr(self.means), _lib.ptr(self.radii), N, W, H, ws, nb, cap,
+                _lib.ptr(self.point_list), _lib.ptr(self.ranges), S()))),
+            ("render_fwd", lambda: _lib.check(L.cvqpg_render_forward_packed(
+                N, C, W, H, ws, nb, cap, _lib.ptr(self.point_list), _lib.ptr(self.ranges),
+                fr(self.field), S()))),
+            ("blasm_fwd", lambda: _lib.check(L.cvqpg_blasm_forward(
+                fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
+                _lib.ptr(self.inten), S()))),
+            ("loss", lambda: _lib.check(L.cvqpg_loss_forward_backward(
+                _lib.ptr(self.inten), _lib.ptr(self.targets), _lib.ptr(self.masks), None, ns,
+                self.seed, _lib.ptr(self.iteration), C, self.D, H, W, _lib.ptr(self.loss_ws),
+                self.loss_ws.numel(), _lib.ptr(self.grad_inten), _lib.ptr(self.loss), S()))),
+            ("blasm_bwd", lambda: _lib.check(L.cvqpg_blasm_backward(
+                fr(self.field), _lib.ptr(self.grad_inten), fr(self.plane_field), *o._common(),
+                _lib.ptr(ows), ows.numel(), fr(self.grad_field), S()))),
+            ("render_bwd", lambda: _lib.check(L.cvqpg_render_backward_tiles(
+                fr(self.grad_field), _lib.ptr(self.k), N, C, W, H, ws, nb, cap,
+                _lib.ptr(self.point_list), _lib.ptr(self.ranges), S()))),
+            ("reduce", lambda: _lib.check(L.cvqpg_raster_backward_raw(
+                _lib.ptr_array(self.params), N, C, W, H, self.dxy2, self.curv_lambda,
+                self.curv_mode, self.curv_nyquist, _lib.ptr(self.iteration), self.warmup, ws, nb,
+                cap, _lib.ptr(self.grads), _lib.ptr(self.sumsq), S()))),
+            ("adan", lambda: _lib.check(L.cvqpg_adan_step(
+                _lib.ptr_array(self.params), _lib.int64s(self.sizes), _lib.ptr(self.grads),
+                _lib.ptr(self.state), _lib.ptr(self.sumsq), self.n_partials,
+                _lib.floats(self.group_lr), self.lr0, 1e-3, self.num_itrs, b1, b2, b3, 1e-8,
+                1.0, self.warmup, 1 if self.curv_mode != 0 else 0, _lib.ptr(self.iteration),
+                S()))),
+        ]
+
+    STAGES = ("prologue", "bin_count", "bin_sort", "render_fwd", "blasm_fwd", "loss",
+              "blasm_bwd", "render_bwd", "reduce", "adan")
+
+    def step(self):
+        for _, fn in self._stages():
+            fn()
+
+    def step_timed(self, events):
+        """Eager step with a HIP event after every stage (events: dict name -> list)."""
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        prev = e0
+        for name, fn in self._stages():
+            fn()
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            events.setdefault(name, []).append((prev, e))
+            prev = e
+
+    # ------------------------------------------------------------------ graphs
+    def capture(self, split=None):
+        """Record one step into HIP graphs.  With ``split`` the step is recorded as three
+        graphs (stages before ``split``, ``split`` alone, stages after) so the split
+        stage can be bracketed by events on the replay stream."""
+        torch.cuda.synchronize(self.dev)
+        stages = self._stages()
+        names = [n for n, _ in stages]
+        if split is None:
+            groups = [stages]
+        else:
+            i = names.index(split)
+            groups = [stages[:i], stages[i:i + 1], stages[i + 1:]]
+        graphs = []
+        side = torch.cuda.Stream(self.dev)
+        for grp in groups:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    for _, fn in grp:
+                        fn()
+            graphs.append(g)
+        torch.cuda.synchronize(self.dev)
+        self.graphs = graphs
+        self.graph = graphs[0] if len(graphs) == 1 else graphs
+        return graphs
+
+    def run(self, split_events=None):
+        """One step: graph replay if captured, else eager.  With a split capture and
+        ``split_events`` (a list), events bracketing the middle graph are appended."""
+        if self.graph is None:
+            self.step()
+            return
+        gs = self.graphs
+        if len(gs) == 1:
+            gs[0].replay()
+            return
+        gs[0].replay()
+        if split_events is not None:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            gs[1].replay()
+            b.record()
+            split_events.append((a, b))
+        else:
+            gs[1].replay()
+        gs[2].replay()
+
+    def grad_views(self):
+        """Raw gradients as tensors shaped like the parameters."""
+        out, off = {}, 0
+        for n, p in zip(PARAM_NAMES, self.params):
+            out[n] = self.grads[off:off + p.numel()].view_as(p)
+            off += p.numel()
+        return out

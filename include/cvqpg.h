@@ -179,6 +179,71 @@ int cvqpg_blasm_reconstruct(const float *phase, const float *amplitude, int chan
                             const double *distances, double pixel_pitch, double aperture_radius,
                             int grid, void *ws, size_t ws_bytes, float *intensity, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Multi-plane loss.  Replaces the PyTorch loss assembly of                  */
+/* train_2d_gaussian_rgb.py:191-197: multiplane_loss_odak.__call__           */
+/* (utils/propagator.py:417-449, weights [1,1,1]) + GaussianLoss             */
+/* (utils/data_utils.py:22-41) per plane, and their autograd backward.       */
+/* ------------------------------------------------------------------------ */
+size_t cvqpg_loss_workspace_size(int channels, int planes, int height, int width);
+
+/* intensity, target, mask [D,C,H,W] float32 (intensity before the clamp to [0,1]).
+ * Target noise for plane d is noise[...] * noise_scale[d] when `noise` is given,
+ * otherwise a Philox4x32-10 normal draw keyed by (seed, *step_counter, plane, pixel);
+ * noise_scale (HOST [D], 1e-3 * (max T_d - min T_d)) NULL disables it.  Writes
+ * grad_intensity [D,C,H,W] = dL/d(intensity) and loss[0] (device) = the summed loss. */
+int cvqpg_loss_forward_backward(const float *intensity, const float *target, const float *mask,
+                                const float *noise, const float *noise_scale, uint64_t seed,
+                                const int32_t *step_counter, int channels, int planes,
+                                int height, int width, void *ws, size_t ws_bytes,
+                                float *grad_intensity, float *loss, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused per-Gaussian prologue and its adjoint (the reference runs these in  */
+/* PyTorch: Gaussians2D.apply_activations / compute_2d_covariance_elements / */
+/* invert_cov_2D, model_2d_gaussian_rgb.py:83-153, and Scene2D._render_cuda  */
+/* :428-436).  raw_params: 7 device pointers in the order means_2d [N,2],    */
+/* pre_act_scales [N,2], pre_act_rotation [N], colours [N,C], pre_act_phase  */
+/* [N,C], pre_act_opacities [N], curvature [N].  curv_mode 0 off, 1 constant */
+/* (200 tanh), 2 scale-aware (curv_nyquist lambda / (6 sigma dxy2)).          */
+/* ------------------------------------------------------------------------ */
+
+/* Activations, covariance, inverse, area and 5-sigma radius; writes means_out [N,2]
+ * and radii_out [N] (for cvqpg_bin_count) and the packed raster record into the raster
+ * workspace, so cvqpg_render_forward_packed / _backward_tiles can follow directly. */
+int cvqpg_prologue_forward(const float *const *raw_params, int n, int channels, int width,
+                           int height, const float *k, float dxy2, float curv_lambda,
+                           int curv_mode, float curv_nyquist, void *ws, size_t ws_bytes,
+                           int64_t capacity, float *means_out, int32_t *radii_out,
+                           void *stream);
+
+/* After cvqpg_render_backward_tiles: per-Gaussian sum of the partial records, adjoint of
+ * the prologue, nan_to_num (train_2d_gaussian_rgb.py:206-208); writes the raw-parameter
+ * gradients flat [(7 + 2C) N] in the order above and one sum of squares per 256
+ * Gaussians (sumsq_partials [ceil(N/256)]).  The curvature gradient is 0 while
+ * *iteration < curv_warmup (frozen parameter). */
+int cvqpg_raster_backward_raw(const float *const *raw_params, int n, int channels, int width,
+                              int height, float dxy2, float curv_lambda, int curv_mode,
+                              float curv_nyquist, const int32_t *iteration, int curv_warmup,
+                              void *ws, size_t ws_bytes, int64_t capacity, float *grads,
+                              float *sumsq_partials, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Optimiser tail: clip_grad_norm_(max_norm) + Adan.step + scheduler.step   */
+/* (train_2d_gaussian_rgb.py:205-211, utils/optimizer.py:105-262,            */
+/* train_2d_gaussian_rgb.py:58-86).  params: the 7 raw tensors (updated in   */
+/* place), sizes[7] their element counts, grads flat as produced above,      */
+/* state flat [4][total] (exp_avg, exp_avg_sq, exp_avg_diff, neg_pre_grad,   */
+/* zero-initialised), group_lr HOST [7] per tensor (entry 0 is replaced by   */
+/* the cosine schedule lr0 -> eta_min over num_itrs).  *iteration (device)   */
+/* is the 0-based iteration and is incremented at the end.                   */
+/* ------------------------------------------------------------------------ */
+int cvqpg_adan_step(float *const *params, const int64_t *sizes, const float *grads, float *state,
+                    const float *sumsq_partials, int n_partials, const float *group_lr,
+                    float lr0, float eta_min, int num_itrs, float beta1, float beta2,
+                    float beta3, float eps, float max_norm, int curv_warmup, int curv_enabled,
+                    int32_t *iteration, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""GPU parity of the fused all-HIP training step against the FP64 oracle step.
+
+Oracle: oracle.step.loss_and_grads -- prologue (model_2d_gaussian_rgb.py:83-153,424-439),
+binned raster, soft_window_pad, glue, BLASM (FP64 transfer-function phase), crop, clamp,
+3-term L2 + SSIM (train_2d_gaussian_rgb.py:187-205) differentiated by torch autograd in
+FP64 -- followed by oracle.adan (utils/optimizer.py:105-262, train:58-86,205-211).
+
+Bars: loss relative error <= 1e-4; raw-parameter gradients relative L2 <= 1e-3 per
+tensor (the end-to-end gradient chains the FP32 raster, two FFT passes each way and the
+FP32 SSIM moments; the per-stage bars of 1e-4 are in test_gpu_raster/blasm/loss);
+Adan: the kernel's update from the step's own gradients equals the oracle's Adan update
+from the same gradients (every element within 2 FP32 ulp of the parameter plus 1e-5 of
+the step, over four steps spanning the curvature unfreeze).  The captured HIP graph
+replays bit-identically to the eager step.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adan as oadan
+from oracle import step as ostep
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+SPEC = dict(W=64, H=48, N=300, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
+            distances=[0.0, 3e-3, 6e-3], kappa=0.05)
+SPEC_GRAY = dict(W=48, H=48, N=200, wavelengths=[515e-9], pitch=8e-6,
+                 distances=[0.0, 2.5e-3, 5e-3], kappa=0.02)
+
+
+def build(spec, seed=0, warmup=0):
+    from cvqpg_amd import fused, train
+    wl = train.Workload("test", device=DEV, seed=seed, spec=spec)
+    raw0 = {n: getattr(wl.gaussians, n).detach().clone() for n in ostep.adan.PARAM_NAMES}
+    tr = fused.FusedTrainer(wl, num_itrs=100, warmup_iters=warmup, inject_noise=False)
+    return wl, tr, raw0
+
+
+def oracle_cfg(wl):
+    ap = wl.args_prop
+    return ostep.StepConfig(wl.img_size, ap.wavelengths, ap.pixel_pitch, ap.pad_size,
+                            ap.distances, aperture_size=ap.aperture_size, grid="odak")
+
+
+@pytest.mark.parametrize("spec", [SPEC, SPEC_GRAY], ids=["rgb", "gray"])
+def test_fused_step_matches_oracle(pkg, spec):
+    wl, tr, raw0 = build(spec)
+    tr.step()
+    torch.cuda.synchronize()
+    loss_gpu = float(tr.loss.item())
+    grads_gpu = {n: g.detach().cpu().double() for n, g in tr.grad_views().items()}
+    raw = {n: v.cpu().double() for n, v in raw0.items()}
+    L, grads, _ = ostep.loss_and_grads(raw, oracle_cfg(wl), wl.targets.cpu().double(),
+                                       wl.masks.cpu().double())
+    grads, _ = oadan.clip_grads(grads, max_norm=float("inf"))  # nan_to_num only
+    el = abs(loss_gpu - float(L)) / abs(float(L))
+    errs = {n: float((grads_gpu[n] - grads[n]).norm() / grads[n].norm()) for n in grads}
+    print(f"loss {loss_gpu:.6f} vs {float(L):.6f} (rel {el:.1e}); grads "
+          + " ".join(f"{n}={e:.1e}" for n, e in errs.items()), flush=True)
+    assert el <= 1e-4
+    for n, e in errs.items():
+        assert e <= 1e-3, (n, e)
+
+
+def test_adan_kernel_matches_oracle(pkg):
+    wl, tr, raw0 = build(SPEC, warmup=2)
+    state = oadan.AdanState(lr0=0.01, num_itrs=100)
+    ref = {n: v.cpu().double() for n, v in raw0.items()}
+    for it in range(4):
+        before = {n: p.detach().cpu().double().clone() for n, p in zip(ostep.adan.PARAM_NAMES, tr.params)}
+        tr.step()
+        torch.cuda.synchronize()
+        g = {n: v.detach().cpu().double() for n, v in tr.grad_views().items()}
+        if it < 2:
+            g["curvature"] = None          # frozen during the warm-up iterations
+        # oracle Adan from the kernel's own (unclipped) gradients and pre-step parameters
+        g, norm = oadan.clip_grads(g)
+        oracle_params = {n: before[n].clone() for n in before}
+        state.step(oracle_params, g)
+        after = {n: p.detach().cpu() for n, p in zip(ostep.adan.PARAM_NAMES, tr.params)}
+        for n in after:
+            a_k = after[n].double()
+            a_o = oracle_params[n]
+            step_o = (a_o - before[n]).abs()
+            # the kernel stores FP32 parameters: allow 2 ulp of the parameter plus 1e-5 of
+            # the step (the increment of an O(1) parameter by lr ~ 1e-3 is itself rounded)
+            ulp = torch.as_tensor(np.spacing(np.abs(after[n].numpy()).astype(np.float32))).double()
+            bad = (a_k - a_o).abs() > 2 * ulp + 1e-5 * step_o
+            assert not bool(bad.any()), (it, n, float((a_k - a_o).abs().max()))
+    assert int(tr.iteration.item()) == 4
+
+
+def test_graph_replay_matches_eager(pkg):
+    from cvqpg_amd import fused, train
+    outs = []
+    for use_graph in (False, True):
+        wl = train.Workload("test", device=DEV, seed=3, spec=SPEC)
+        tr = fused.FusedTrainer(wl, num_itrs=100, warmup_iters=1)
+        tr.step()                        # eager warm-up step
+        if use_graph:
+            tr.capture()                 # recording does not execute the step
+        losses = []
+        for _ in range(4):
+            tr.run()
+            torch.cuda.synchronize()
+            losses.append(float(tr.loss.item()))
+        outs.append((losses, [p.detach().clone() for p in tr.params]))
+    (l0, p0), (l1, p1) = outs
+    print(f"eager {l0} graph {l1}", flush=True)
+    assert l0 == l1
+    for a, b in zip(p0, p1):
+        assert torch.equal(a, b)
