@@ -13,7 +13,8 @@ import os
 import torch  # noqa: F401  (must precede the library load)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcvqpg.so")
+# CVQPG_LIB may point at another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("CVQPG_LIB", os.path.join(HERE, "libcvqpg.so"))
 
 _c_int, _c_i64, _c_f32, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
 _c_f64 = ctypes.c_double

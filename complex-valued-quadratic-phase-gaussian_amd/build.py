@@ -32,9 +32,9 @@ def sources():
                   if f.endswith((".hip", ".cpp")))
 
 
-def _compile(src, verbose):
-    obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj] + COMMON_FLAGS
+def _compile(src, verbose, bdir, extra=()):
+    obj = os.path.join(bdir, os.path.basename(src) + ".o")
+    cmd = [HIPCC, "-x", "hip", "-c", src, "-o", obj] + COMMON_FLAGS + list(extra)
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,24 +43,29 @@ def _compile(src, verbose):
     return obj
 
 
-def build_library(verbose=False, jobs=None):
-    """Compile every source for gfx950 and link libcvqpg.so; returns its path."""
+def build_library(verbose=False, jobs=None, out=None, build_dir=None, defines=()):
+    """Compile every source for gfx950 and link libcvqpg.so; returns its path.
+    ``defines`` are extra -D flags (tuning constants); ``out``/``build_dir`` redirect the
+    result."""
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(BUILD_DIR, exist_ok=True)
+    lib_path = out or LIB_PATH
+    bdir = build_dir or BUILD_DIR
+    os.makedirs(bdir, exist_ok=True)
+    extra = [f"-D{d}" for d in defines]
     srcs = sources()
     jobs = jobs or min(8, len(srcs))
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    tmp = LIB_PATH + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, verbose, bdir, extra), srcs))
+    tmp = lib_path + ".tmp"
     cmd = [HIPCC, "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 def clean():

@@ -29,9 +29,16 @@
 namespace cvqpg {
 
 constexpr int MAXD = 16;
-constexpr int ROW_NT = 512;
+#ifndef CVQPG_ROW_NT
+#define CVQPG_ROW_NT 256
+#endif
+#ifndef CVQPG_COL_NT
+#define CVQPG_COL_NT 512
+#endif
+constexpr int ROW_NT = CVQPG_ROW_NT;
 constexpr int ROW_LCAP = 8704;   // longest row transform (complex elements)
-constexpr int COL_NT = 1024;
+constexpr int ROW_LCAP_SMALL = 4608;
+constexpr int COL_NT = CVQPG_COL_NT;
 constexpr int COL_LCAP = 6144;   // column length x columns per workgroup
 constexpr int COL_PER = COL_LCAP / COL_NT;
 
@@ -43,20 +50,21 @@ struct Optics {
   int gr0;               // first 2x-grid row of the hologram (adjoint output rows)
   int cg;                // columns per column workgroup
   float norm;            // 1 / (nu nv)
-  double fr0, frs;       // centred row-frequency grid  f = fr0 + i frs
-  double fc0, fcs;       // centred column-frequency grid
-  double inv_lam2[3];
-  double z[MAXD];
-  float fxmax[MAXD][3];   // band limit on |FX| (column frequency)
-  float fymax[MAXD][3];   // band limit on |FY| (row frequency)
+  float fr0, frs;        // centred row-frequency grid  f = fr0 + i frs  (1/m)
+  float fc0, fcs;        // centred column-frequency grid
+  float lam2[3];         // lambda^2
+  float zl[MAXD][3];     // z / lambda (revolutions of the carrier)
+  float2 carrier[MAXD][3];  // exp(i 2 pi z / lambda), reduced in FP64 on the host
+  float fxmax[MAXD][3];  // band limit on |FX| (column frequency)
+  float fymax[MAXD][3];  // band limit on |FY| (row frequency)
   int ap_open;
-  int band_active;         // 0 when no band limit and no evanescence can occur on the grid
-  double ap_r2, apr0, aprs, apc0, apcs;
+  int band_active;       // 0 when no band limit and no evanescence can occur on the grid
+  float ap_r2, apr0, aprs, apc0, apcs;
 };
 
 struct BlasmWS {
-  float2* tw_r;   // [nu]
-  float2* tw_c;   // [nv]
+  float2* tw_r;   // two-level twiddle table of the column length nu
+  float2* tw_c;   // two-level twiddle table of the row length nv
   float2* X;      // [C][pH][nv]   (forward pass 1 output; adjoint pass 2 output [C][H][nv])
   float2* Y;      // [D][C][oh][nv] (forward pass 2 output; adjoint pass 3 output)
   size_t bytes;
@@ -66,8 +74,8 @@ static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int 
   Carver cv(base);
   BlasmWS w;
   const size_t nu = 2 * (size_t)pH, nv = 2 * (size_t)pW;
-  w.tw_r = cv.take<float2>(nu);
-  w.tw_c = cv.take<float2>(nv);
+  w.tw_r = cv.take<float2>(fft::twiddle_entries((int)nu));
+  w.tw_c = cv.take<float2>(fft::twiddle_entries((int)nv));
   const size_t xr = (size_t)(pH > H ? pH : H);
   w.X = cv.take<float2>((size_t)C * xr * nv);
   w.Y = cv.take<float2>((size_t)D * C * oh * nv);
@@ -75,56 +83,68 @@ static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int 
   return w;
 }
 
+// two-level twiddle table of length n: W_n^{64 h} for h < nh, then W_n^l for l < 64
 __global__ void k_twiddles(float2* __restrict__ tw, int n) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
+  const int nh = (n + fft::TW_LO - 1) / fft::TW_LO;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nh + fft::TW_LO) return;
+  const long m = i < nh ? (long)i * fft::TW_LO : (long)(i - nh);
   double s, c;
   sincospi(2.0 * (double)m / (double)n, &s, &c);
-  tw[m] = make_float2((float)c, (float)-s);
+  tw[i] = make_float2((float)c, (float)-s);
+}
+
+// copy the plan's twiddle table into LDS; the caller's next barrier publishes it
+__device__ __forceinline__ const float2* stage_twiddles(const fft::Plan& p, float2* stw) {
+  const int ne = p.nh + fft::TW_LO;
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) stw[i] = p.tw[i];
+  return stw;
 }
 
 // ---------------------------------------------------------------------------------
 // transfer function
 // ---------------------------------------------------------------------------------
-// Per spectral point the column passes keep only sqrt(1/lambda^2 - FX^2 - FY^2) in FP64
-// (negative: no transmission at any plane); the band-limit test is re-evaluated per
-// plane only when the host found that it can remove anything on this grid.
+// Transfer function.  With x = lambda^2 (FX^2 + FY^2),
+//   2 pi z sqrt(1/lambda^2 - FX^2 - FY^2) = 2 pi z/lambda - 2 pi (z/lambda) g(x),
+//   g(x) = 1 - sqrt(1 - x) = x / (1 + sqrt(1 - x)).
+// The first term is a constant carrier phase per (plane, channel); it is reduced modulo
+// one revolution in FP64 on the host and applied to the stored plane field.  The second
+// term is at most a few revolutions on the grids used here, so FP32 evaluates it to
+// ~1e-6 revolution (an FP32 evaluation of the full phase of ~1e4 revolutions would be off
+// by ~1e-3 revolution).  Per spectral point the column passes keep only g (negative: no
+// transmission, evanescent or outside the aperture).
 __device__ __forceinline__ void tf_freqs(const Optics& o, int kr, int kc, int& ir, int& ic,
-                                         double& fy, double& fx) {
+                                         float& fy, float& fx) {
   // natural FFT index -> centred grid index (the ifftshift of utils/propagator.py:63)
   ir = kr + (o.nu >> 1) >= o.nu ? kr + (o.nu >> 1) - o.nu : kr + (o.nu >> 1);
   ic = kc + (o.nv >> 1) >= o.nv ? kc + (o.nv >> 1) - o.nv : kc + (o.nv >> 1);
-  fy = o.fr0 + (double)ir * o.frs;
-  fx = o.fc0 + (double)ic * o.fcs;
+  fy = fmaf((float)ir, o.frs, o.fr0);
+  fx = fmaf((float)ic, o.fcs, o.fc0);
 }
 
-__device__ __forceinline__ double tf_point(const Optics& o, int c, int kr, int kc) {
+__device__ __forceinline__ float tf_point(const Optics& o, int c, int kr, int kc) {
   int ir, ic;
-  double fy, fx;
+  float fy, fx;
   tf_freqs(o, kr, kc, ir, ic, fy, fx);
-  const double rad = o.inv_lam2[c] - (fx * fx + fy * fy);
-  double sq = rad >= 0.0 ? sqrt(rad) : -1.0;
+  const float x = o.lam2[c] * fmaf(fx, fx, fy * fy);
+  float g = x <= 1.0f ? x / (1.0f + sqrtf(1.0f - x)) : -1.0f;
   if (!o.ap_open) {
-    const double X = o.apr0 + (double)ir * o.aprs, Y = o.apc0 + (double)ic * o.apcs;
-    if (!(X * X + Y * Y < o.ap_r2)) sq = -1.0;
+    const float X = fmaf((float)ir, o.aprs, o.apr0), Y = fmaf((float)ic, o.apcs, o.apc0);
+    if (!(X * X + Y * Y < o.ap_r2)) g = -1.0f;
   }
-  return sq;
+  return g;
 }
 
-// H_d at a point: exp(+i 2 pi z sqrt(...)) inside the band, 0 outside; the phase is
-// reduced modulo one revolution in FP64 before the FP32 sin/cos
-__device__ __forceinline__ float2 tf_value(const Optics& o, double sq, int c, int d, int kr,
-                                           int kc) {
-  if (sq < 0.0) return make_float2(0.f, 0.f);
+// H_d / carrier at a point: exp(-i 2 pi (z/lambda) g) inside the band, 0 outside
+__device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int d, int kr, int kc) {
+  if (g < 0.0f) return make_float2(0.f, 0.f);
   if (o.band_active) {
     int ir, ic;
-    double fy, fx;
+    float fy, fx;
     tf_freqs(o, kr, kc, ir, ic, fy, fx);
-    if (!((float)fabs(fx) < o.fxmax[d][c]) || !((float)fabs(fy) < o.fymax[d][c]))
-      return make_float2(0.f, 0.f);
+    if (!(fabsf(fx) < o.fxmax[d][c]) || !(fabsf(fy) < o.fymax[d][c])) return make_float2(0.f, 0.f);
   }
-  const double rev = o.z[d] * sq;
-  const float fr = (float)(rev - floor(rev));
+  const float fr = __builtin_amdgcn_fractf(-o.zl[d][c] * g);
   return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
 }
 
@@ -167,11 +187,12 @@ __device__ __forceinline__ float taper(int t, int total, int inner_start, int in
 // row passes
 // ---------------------------------------------------------------------------------
 // F1: padded/tapered/glued row -> FFT along the 2x row.  grid (pH, C)
-template <int MAXR>
+template <int MAXR, int LCAP>
 __global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ field,
                                                     Optics o, fft::Plan plan,
                                                     float2* __restrict__ X) {
   extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
   const int r = blockIdx.x, c = blockIdx.y;
   const int sr = min(max(r - o.pad_h, 0), o.H - 1);
   const float wy = taper(r, o.pH, o.pad_h, o.H);
@@ -185,21 +206,22 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ f
       const float2 h = src[sc];
       u = glue(make_float2(h.x * w, h.y * w));
     }
-    buf[j] = u;
+    buf[fft::lidx(j)] = u;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
   float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // F1 for the propagator interface: u = amplitude e^{i phase} on [C, pH, pW].  grid (pH, C)
-template <int MAXR>
+template <int MAXR, int LCAP>
 __global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restrict__ phase,
                                                           const float* __restrict__ amp,
                                                           Optics o, fft::Plan plan,
                                                           float2* __restrict__ X) {
   extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
   const int r = blockIdx.x, c = blockIdx.y;
   const size_t base = ((size_t)c * o.pH + r) * o.pW;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
@@ -211,76 +233,82 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restric
       sincosf(phase[base + q], &s, &co);
       u = make_float2(a * co, a * s);
     }
-    buf[j] = u;
+    buf[fft::lidx(j)] = u;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
   float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // F3: inverse FFT along the row, crop, field and intensity.  grid (oh, D*C)
-template <int MAXR>
+template <int MAXR, int LCAP>
 __global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y, Optics o,
                                                     fft::Plan plan, float2* __restrict__ out,
                                                     float* __restrict__ inten) {
   extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
   const int r = blockIdx.x, dc = blockIdx.y;
   const float2* src = Y + ((size_t)dc * o.oh + r) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[j] = src[j];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
   __syncthreads();
-  fft::run<-1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  fft::run<-1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
   const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  const float2 cr = o.carrier[dc / o.C][dc % o.C];
   for (int x = threadIdx.x; x < o.ow; x += ROW_NT) {
-    const float2 v = buf[o.oc0 + x];
+    const float2 v = buf[fft::lidx(o.oc0 + x)];
     const float2 f = make_float2(v.x * o.norm, v.y * o.norm);
-    if (out) out[ob + x] = f;
+    if (out) out[ob + x] = fft::cmul(f, cr);
     inten[ob + x] = f.x * f.x + f.y * f.y;
   }
 }
 
 // A3: adjoint of F3.  g_out = 2 dL/dI * out (times the 1/(nu nv) of the forward), placed in
 // the cropped columns of a zero 2x row, forward FFT.  grid (oh, D*C)
-template <int MAXR>
+template <int MAXR, int LCAP>
 __global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ grad_inten,
                                                     const float2* __restrict__ out, Optics o,
                                                     fft::Plan plan, float2* __restrict__ Z) {
   extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
   const int r = blockIdx.x, dc = blockIdx.y;
   const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  const float2 cr = o.carrier[dc / o.C][dc % o.C];
+  const float2 crc = make_float2(cr.x, -cr.y);
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
     const int x = j - o.oc0;
     float2 g = make_float2(0.f, 0.f);
     if (x >= 0 && x < o.ow) {
       const float s = 2.0f * grad_inten[ob + x] * o.norm;
-      const float2 f = out[ob + x];
+      const float2 f = fft::cmul(out[ob + x], crc);
       g = make_float2(s * f.x, s * f.y);
     }
-    buf[j] = g;
+    buf[fft::lidx(j)] = g;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
   float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[j];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // A1: adjoint of F1 on the hologram rows: unnormalised inverse FFT along the row, the
 // hologram columns, adjoint of the glue.  grid (H, C)
-template <int MAXR>
+template <int MAXR, int LCAP>
 __global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G,
                                                     const float2* __restrict__ field, Optics o,
                                                     fft::Plan plan,
                                                     float2* __restrict__ grad_field) {
   extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
   const int y = blockIdx.x, c = blockIdx.y;
   const float2* src = G + ((size_t)c * o.H + y) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[j] = src[j];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
   __syncthreads();
-  fft::run<-1, 1, ROW_NT, ROW_LCAP, MAXR>(buf, plan);
+  fft::run<-1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
   const size_t fb = ((size_t)c * o.H + y) * o.W;
   const int c0 = o.zc0 + o.pad_w;
   for (int x = threadIdx.x; x < o.W; x += ROW_NT)
-    grad_field[fb + x] = glue_adjoint(field[fb + x], buf[c0 + x]);
+    grad_field[fb + x] = glue_adjoint(field[fb + x], buf[fft::lidx(c0 + x)]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -301,21 +329,22 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
   const int c = blockIdx.y;
   const int n = o.nu;
   const int total = n * CG;
+  const float2* stw = stage_twiddles(plan_fwd, buf + fft::lds_elems(total));
   for (int e = threadIdx.x; e < total; e += COL_NT) {
     const int i = e / CG, g = e % CG;
     const int r = i - o.zr0;
-    buf[e] = (r >= 0 && r < o.pH) ? X[((size_t)c * o.pH + r) * o.nv + col0 + g]
+    buf[fft::lidx(e)] = (r >= 0 && r < o.pH) ? X[((size_t)c * o.pH + r) * o.nv + col0 + g]
                                   : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd);
+  fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd, stw);
   float2 S[COL_PER];
-  double P[COL_PER];
+  float P[COL_PER];
 #pragma unroll
   for (int b = 0; b < COL_PER; ++b) {
     const int e = threadIdx.x + b * COL_NT;
     if (e < total) {
-      S[b] = buf[e];
+      S[b] = buf[fft::lidx(e)];
       P[b] = tf_point(o, c, e / CG, col0 + e % CG);
     }
   }
@@ -324,14 +353,14 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
 #pragma unroll
     for (int b = 0; b < COL_PER; ++b) {
       const int e = threadIdx.x + b * COL_NT;
-      if (e < total) buf[e] = fft::cmul(S[b], tf_value(o, P[b], c, d, e / CG, col0 + e % CG));
+      if (e < total) buf[fft::lidx(e)] = fft::cmul(S[b], tf_value(o, P[b], c, d, e / CG, col0 + e % CG));
     }
     __syncthreads();
-    fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd);
+    fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd, stw);
     float2* dst = Y + ((size_t)d * o.C + c) * o.oh * o.nv;
     for (int e = threadIdx.x; e < o.oh * CG; e += COL_NT) {
       const int i = e / CG, g = e % CG;
-      dst[(size_t)i * o.nv + col0 + g] = buf[(o.or0 + i) * CG + g];
+      dst[(size_t)i * o.nv + col0 + g] = buf[fft::lidx((o.or0 + i) * CG + g)];
     }
   }
 }
@@ -348,8 +377,9 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
   if (col0 >= o.nv) return;
   const int c = blockIdx.y;
   const int total = o.nu * CG;
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(total));
   float2 A[COL_PER];
-  double P[COL_PER];
+  float P[COL_PER];
 #pragma unroll
   for (int b = 0; b < COL_PER; ++b) {
     const int e = threadIdx.x + b * COL_NT;
@@ -361,16 +391,16 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
     for (int e = threadIdx.x; e < total; e += COL_NT) {
       const int i = e / CG, g = e % CG;
       const int r = i - o.or0;
-      buf[e] = (r >= 0 && r < o.oh) ? src[(size_t)r * o.nv + col0 + g] : make_float2(0.f, 0.f);
+      buf[fft::lidx(e)] = (r >= 0 && r < o.oh) ? src[(size_t)r * o.nv + col0 + g] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan);
+    fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan, stw);
 #pragma unroll
     for (int b = 0; b < COL_PER; ++b) {
       const int e = threadIdx.x + b * COL_NT;
       if (e < total) {
         const float2 h = tf_value(o, P[b], c, d, e / CG, col0 + e % CG);
-        const float2 v = buf[e];
+        const float2 v = buf[fft::lidx(e)];
         A[b].x += v.x * h.x + v.y * h.y;  // v * conj(h)
         A[b].y += v.y * h.x - v.x * h.y;
       }
@@ -380,14 +410,14 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
 #pragma unroll
   for (int b = 0; b < COL_PER; ++b) {
     const int e = threadIdx.x + b * COL_NT;
-    if (e < total) buf[e] = A[b];
+    if (e < total) buf[fft::lidx(e)] = A[b];
   }
   __syncthreads();
-  fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan);
+  fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan, stw);
   float2* dst = G + (size_t)c * o.H * o.nv;
   for (int e = threadIdx.x; e < o.H * CG; e += COL_NT) {
     const int i = e / CG, g = e % CG;
-    dst[(size_t)i * o.nv + col0 + g] = buf[(o.gr0 + i) * CG + g];
+    dst[(size_t)i * o.nv + col0 + g] = buf[fft::lidx((o.gr0 + i) * CG + g)];
   }
 }
 
@@ -395,13 +425,37 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
 // host
 // ---------------------------------------------------------------------------------
 // row kernels are instantiated for radices <= 8 (lean) and <= 17
-#define ROW_LAUNCH(K, PLAN, ...)                                  \
-  do {                                                            \
-    if (fft::max_radix(PLAN) <= 8)                                \
-      hipLaunchKernelGGL(K<8>, __VA_ARGS__);                      \
-    else                                                          \
-      hipLaunchKernelGGL(K<17>, __VA_ARGS__);                     \
+// row kernels are instantiated per radix class (<= 8, <= 17) and length class
+// (<= 4608, <= 8704) so the per-thread butterfly registers match the transform
+#define ROW_LAUNCH(K, PLAN, GRID, SH, STREAM, ...)                                   \
+  do {                                                                               \
+    const bool small_ = (PLAN).n <= ROW_LCAP_SMALL;                                  \
+    if (fft::max_radix(PLAN) <= 8) {                                                 \
+      if (small_) launch_row(K<8, ROW_LCAP_SMALL>, GRID, SH, STREAM, __VA_ARGS__);   \
+      else launch_row(K<8, ROW_LCAP>, GRID, SH, STREAM, __VA_ARGS__);                \
+    } else {                                                                         \
+      if (small_) launch_row(K<17, ROW_LCAP_SMALL>, GRID, SH, STREAM, __VA_ARGS__);  \
+      else launch_row(K<17, ROW_LCAP>, GRID, SH, STREAM, __VA_ARGS__);               \
+    }                                                                                \
   } while (0)
+
+// dynamic LDS above 64 KB must be enabled per kernel
+template <class Kern>
+static void allow_lds(Kern k, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <class Kern, class... Args>
+static void launch_row(Kern k, dim3 grid, size_t sh, hipStream_t s, Args... args) {
+  allow_lds(k, sh);
+  hipLaunchKernelGGL(k, grid, dim3(ROW_NT), sh, s, args...);
+}
+
+static size_t row_lds_bytes(const Optics& o) {
+  return (size_t)(fft::lds_elems(o.nv) + fft::twiddle_entries(o.nv)) * sizeof(float2);
+}
 
 static int pick_cg(int nu, int nv) {
   for (int cg = 8; cg >= 1; cg >>= 1)
@@ -437,20 +491,22 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length", pH);
   o.norm = (float)(1.0 / ((double)o.nu * (double)o.nv));
   const double X = pitch * o.nu, Y = pitch * o.nv;
+  double fr0, frs, fc0, fcs;
   if (grid == 0) {  // odak: linspace(-1/(2dx) + 0.5/(2X), 1/(2dx) - 0.5/(2X), n)
-    o.fr0 = -1.0 / (2 * pitch) + 0.5 / (2 * X);
-    o.frs = (1.0 / pitch - 0.5 / X) / (o.nu - 1);
-    o.fc0 = -1.0 / (2 * pitch) + 0.5 / (2 * Y);
-    o.fcs = (1.0 / pitch - 0.5 / Y) / (o.nv - 1);
+    fr0 = -1.0 / (2 * pitch) + 0.5 / (2 * X);
+    frs = (1.0 / pitch - 0.5 / X) / (o.nu - 1);
+    fc0 = -1.0 / (2 * pitch) + 0.5 / (2 * Y);
+    fcs = (1.0 / pitch - 0.5 / Y) / (o.nv - 1);
   } else {          // exact FFT bins (i - n/2) / X
-    o.fr0 = -(double)(o.nu / 2) / X;
-    o.frs = 1.0 / X;
-    o.fc0 = -(double)(o.nv / 2) / Y;
-    o.fcs = 1.0 / Y;
+    fr0 = -(double)(o.nu / 2) / X;
+    frs = 1.0 / X;
+    fc0 = -(double)(o.nv / 2) / Y;
+    fcs = 1.0 / Y;
   }
+  o.fr0 = (float)fr0; o.frs = (float)frs; o.fc0 = (float)fc0; o.fcs = (float)fcs;
   for (int c = 0; c < 3; ++c) {
-    const double lam = c < C ? wavelengths[c] : 1.0;
-    o.inv_lam2[c] = 1.0 / (lam * lam);
+    const double lam = c < C ? wavelengths[c] : 0.0;
+    o.lam2[c] = (float)(lam * lam);
   }
   for (int d = 0; d < MAXD; ++d)
     for (int c = 0; c < 3; ++c) {
@@ -458,17 +514,23 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
         const double z = distances[d], lam = wavelengths[c];
         o.fxmax[d][c] = (float)(1.0 / sqrt((2 * z / X) * (2 * z / X) + 1) / lam);
         o.fymax[d][c] = (float)(1.0 / sqrt((2 * z / Y) * (2 * z / Y) + 1) / lam);
+        const double zl = z / lam;
+        o.zl[d][c] = (float)zl;
+        const double fr = zl - floor(zl);
+        o.carrier[d][c] = make_float2((float)cos(2 * M_PI * fr), (float)sin(2 * M_PI * fr));
       } else {
         o.fxmax[d][c] = o.fymax[d][c] = 0.0f;
+        o.zl[d][c] = 0.0f;
+        o.carrier[d][c] = make_float2(1.0f, 0.0f);
       }
     }
-  for (int d = 0; d < MAXD; ++d) o.z[d] = d < D ? distances[d] : 0.0;
   {
-    const double fymaxabs = fmax(fabs(o.fr0), fabs(o.fr0 + (o.nu - 1) * o.frs));
-    const double fxmaxabs = fmax(fabs(o.fc0), fabs(o.fc0 + (o.nv - 1) * o.fcs));
+    const double fymaxabs = fmax(fabs(fr0), fabs(fr0 + (o.nu - 1) * frs));
+    const double fxmaxabs = fmax(fabs(fc0), fabs(fc0 + (o.nv - 1) * fcs));
     o.band_active = 0;
     for (int c = 0; c < C; ++c) {
-      if (o.inv_lam2[c] - (fxmaxabs * fxmaxabs + fymaxabs * fymaxabs) < 0) o.band_active = 1;
+      const double lam = wavelengths[c];
+      if (lam * lam * (fxmaxabs * fxmaxabs + fymaxabs * fymaxabs) >= 1.0) o.band_active = 1;
       for (int d = 0; d < D; ++d)
         if (!((float)fxmaxabs < o.fxmax[d][c]) || !((float)fymaxabs < o.fymax[d][c]))
           o.band_active = 1;
@@ -477,11 +539,11 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   // circular aperture on the centred grid: linspace(-n/2, n/2, n) per axis
   const double rmax = sqrt((o.nu / 2.0) * (o.nu / 2.0) + (o.nv / 2.0) * (o.nv / 2.0));
   o.ap_open = !(aperture_radius > 0) || aperture_radius > rmax + 1.0;
-  o.ap_r2 = aperture_radius * aperture_radius;
-  o.apr0 = -o.nu / 2.0;
-  o.aprs = (double)o.nu / (o.nu - 1);
-  o.apc0 = -o.nv / 2.0;
-  o.apcs = (double)o.nv / (o.nv - 1);
+  o.ap_r2 = (float)(aperture_radius * aperture_radius);
+  o.apr0 = (float)(-o.nu / 2.0);
+  o.aprs = (float)((double)o.nu / (o.nu - 1));
+  o.apc0 = (float)(-o.nv / 2.0);
+  o.apcs = (float)((double)o.nv / (o.nv - 1));
   return 0;
 }
 
@@ -489,28 +551,28 @@ template <int CG>
 static void launch_f2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
   const int nb = o.nv / CG, per = ceil_div(nb, 8);
   const dim3 g(8 * per, o.C), b(COL_NT);
-  const size_t sh = (size_t)o.nu * CG * sizeof(float2);
+  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
   const int mr = fft::max_radix(pr);
   if (mr <= 8)
-    hipLaunchKernelGGL((k_f2_cols<CG, 8>), g, b, sh, s, w.X, o, pr, per, w.Y);
+    { allow_lds(k_f2_cols<CG, 8>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 8>), g, b, sh, s, w.X, o, pr, per, w.Y); }
   else if (mr <= 13)
-    hipLaunchKernelGGL((k_f2_cols<CG, 13>), g, b, sh, s, w.X, o, pr, per, w.Y);
+    { allow_lds(k_f2_cols<CG, 13>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 13>), g, b, sh, s, w.X, o, pr, per, w.Y); }
   else
-    hipLaunchKernelGGL((k_f2_cols<CG, 17>), g, b, sh, s, w.X, o, pr, per, w.Y);
+    { allow_lds(k_f2_cols<CG, 17>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 17>), g, b, sh, s, w.X, o, pr, per, w.Y); }
 }
 
 template <int CG>
 static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
   const int nb = o.nv / CG, per = ceil_div(nb, 8);
   const dim3 g(8 * per, o.C), b(COL_NT);
-  const size_t sh = (size_t)o.nu * CG * sizeof(float2);
+  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
   const int mr = fft::max_radix(pr);
   if (mr <= 8)
-    hipLaunchKernelGGL((k_a2_cols<CG, 8>), g, b, sh, s, w.Y, o, pr, per, w.X);
+    { allow_lds(k_a2_cols<CG, 8>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 8>), g, b, sh, s, w.Y, o, pr, per, w.X); }
   else if (mr <= 13)
-    hipLaunchKernelGGL((k_a2_cols<CG, 13>), g, b, sh, s, w.Y, o, pr, per, w.X);
+    { allow_lds(k_a2_cols<CG, 13>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 13>), g, b, sh, s, w.Y, o, pr, per, w.X); }
   else
-    hipLaunchKernelGGL((k_a2_cols<CG, 17>), g, b, sh, s, w.Y, o, pr, per, w.X);
+    { allow_lds(k_a2_cols<CG, 17>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 17>), g, b, sh, s, w.Y, o, pr, per, w.X); }
 }
 
 static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc,
@@ -519,8 +581,8 @@ static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc
   CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "FFT length %d has a prime factor above 17", o.nv);
   pr.tw = w.tw_r;
   pc.tw = w.tw_c;
-  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(o.nu, 256)), dim3(256), 0, s, w.tw_r, o.nu);
-  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(o.nv, 256)), dim3(256), 0, s, w.tw_c, o.nv);
+  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(fft::twiddle_entries(o.nu), 256)), dim3(256), 0, s, w.tw_r, o.nu);
+  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(fft::twiddle_entries(o.nv), 256)), dim3(256), 0, s, w.tw_c, o.nv);
   return 0;
 }
 
@@ -533,8 +595,7 @@ static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan&
     case 2: launch_f2<2>(w, o, pr, s); break;
     default: launch_f2<1>(w, o, pr, s); break;
   }
-  ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), dim3(ROW_NT),
-                     (size_t)o.nv * sizeof(float2), s, w.Y, o, pc,
+  ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, w.Y, o, pc,
                      reinterpret_cast<float2*>(out_field), intensity);
   CVQPG_LAUNCH_CHECK();
   return 0;
@@ -568,7 +629,7 @@ extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height,
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), dim3(ROW_NT), (size_t)o.nv * sizeof(float2), s,
+  ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s,
                      reinterpret_cast<const float2*>(field), o, pc, w.X);
   CVQPG_LAUNCH_CHECK();
   return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
@@ -592,8 +653,7 @@ extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intens
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), dim3(ROW_NT),
-                     (size_t)o.nv * sizeof(float2), s, grad_intensity,
+  ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity,
                      reinterpret_cast<const float2*>(out_field), o, pc, w.Y);
   switch (o.cg) {
     case 8: launch_a2<8>(w, o, pr, s); break;
@@ -601,7 +661,7 @@ extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intens
     case 2: launch_a2<2>(w, o, pr, s); break;
     default: launch_a2<1>(w, o, pr, s); break;
   }
-  ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), dim3(ROW_NT), (size_t)o.nv * sizeof(float2), s,
+  ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s,
                      w.X, reinterpret_cast<const float2*>(field), o, pc,
                      reinterpret_cast<float2*>(grad_field));
   CVQPG_LAUNCH_CHECK();
@@ -625,8 +685,7 @@ extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitud
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), dim3(ROW_NT),
-                     (size_t)o.nv * sizeof(float2), s, phase, amplitude, o, pc, w.X);
+  ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, phase, amplitude, o, pc, w.X);
   CVQPG_LAUNCH_CHECK();
   return forward_cols_rows(w, o, pr, pc, nullptr, intensity, s);
 }

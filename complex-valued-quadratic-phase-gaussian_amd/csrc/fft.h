@@ -11,8 +11,9 @@
 // inputs for a stage are read into registers before the barrier and written after it,
 // so a single LDS buffer suffices.
 //
-// Twiddles come from a table W_n^m = exp(-2 pi i m / n), m in [0, n), evaluated in FP64
-// and rounded to FP32 once.  DIR = +1 is the forward transform (e^{-i}), DIR = -1 the
+// Twiddles come from a two-level table (W_n^{64 h} and W_n^l, l < 64) evaluated in FP64,
+// rounded to FP32 once and staged in LDS by the caller; a butterfly forms W^{r k} as
+// powers of W^k.  DIR = +1 is the forward transform (e^{-i}), DIR = -1 the
 // unnormalised inverse (e^{+i}).  Radices 2, 3, 4, 5 and 8 have dedicated butterflies;
 // the odd primes 7, 11, 13 and 17 use a symmetric direct DFT with compile-time roots.
 #pragma once
@@ -27,8 +28,17 @@ struct Plan {
   int n;
   int nstages;
   int radix[MAX_STAGES];
-  const float2* tw;  // [n]
+  int nh;            // ceil(n / 64)
+  const float2* tw;  // two-level table: tw[h] = W_n^{64 h} (h < nh), tw[nh + l] = W_n^l (l < 64)
 };
+
+constexpr int TW_LO = 64;
+inline int twiddle_entries(int n) { return (n + TW_LO - 1) / TW_LO + TW_LO; }
+
+// LDS element index with one padding slot every 16 complex elements, so the strided
+// accesses of the early Stockham stages spread over the banks.
+__device__ __forceinline__ int lidx(int i) { return i + (i >> 4); }
+__host__ __device__ inline int lds_elems(int n) { return n + (n >> 4) + 1; }
 
 // Factorisation into supported radices; powers of two become radix-8 and radix-4
 // stages (never radix 2 unless n has a single factor 2); returns false if n has a
@@ -36,6 +46,7 @@ struct Plan {
 inline bool make_plan(int n, Plan& p) {
   p.n = n;
   p.nstages = 0;
+  p.nh = (n + TW_LO - 1) / TW_LO;
   if (n < 2) return false;
   int m = n, a = 0;
   while (m % 2 == 0) {
@@ -80,9 +91,11 @@ __device__ __forceinline__ float2 rot90(float2 a) {
   return DIR > 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
 
+// W_n^m from the two-level table (one complex product of two FP32-rounded roots)
 template <int DIR>
-__device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int m) {
-  const float2 w = tw[m];
+__device__ __forceinline__ float2 twid(const float2* tw, int nh, int m) {
+  const float2 a = tw[m >> 6], b = tw[nh + (m & (TW_LO - 1))];
+  const float2 w = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
   return DIR > 0 ? w : make_float2(w.x, -w.y);
 }
 
@@ -244,10 +257,11 @@ __device__ __forceinline__ void dft(float2* v) {
   }
 }
 
-// One radix-R stage on CG interleaved sequences of length n held in buf.
-// LCAP is the LDS capacity in complex elements (n * CG <= LCAP); NT threads.
+// One radix-R stage on CG interleaved sequences of length n held in buf (padded index).
+// LCAP is the LDS capacity in complex elements (n * CG <= LCAP); NT threads.  The twiddles
+// W^{r k} of a butterfly are built as powers of W^k.
 template <int R, int DIR, int CG, int NT, int LCAP>
-__device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* __restrict__ tw) {
+__device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* tw, int nh) {
   constexpr int MAXB = (LCAP / R + NT - 1) / NT;
   const int cnt = n / R;
   const int total = cnt * CG;
@@ -263,10 +277,16 @@ __device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* 
       const int j = w / CG;
       const int k = j % ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * cnt) * CG + col];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[lidx((j + r * cnt) * CG + col)];
       if (ns > 1) {
+        const float2 w1 = twid<DIR>(tw, nh, k * tstep);
+        float2 wr = w1;
+        v[b][1] = cmul(v[b][1], w1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twid<DIR>(tw, r * k * tstep));
+        for (int r = 2; r < R; ++r) {
+          wr = cmul(wr, w1);
+          v[b][r] = cmul(v[b][r], wr);
+        }
       }
       dft<R, DIR>(v[b]);
       dst[b] = ((j / ns) * ns * R + k) * CG + col;
@@ -277,35 +297,36 @@ __device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* 
   for (int b = 0; b < MAXB; ++b) {
     if (dst[b] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[dst[b] + r * ns * CG] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[lidx(dst[b] + r * ns * CG)] = v[b][r];
     }
   }
   __syncthreads();
 }
 
-// Full transform; the caller has written the input to buf and synchronised.  MAXR bounds
+// Full transform; the caller has written the input to buf, staged the twiddle table at tw
+// (LDS) and synchronised.  MAXR bounds
 // the radices compiled into this instance: kernels whose plans only use radices <= 8
 // instantiate MAXR = 8 and do not pay the registers of the large direct-DFT stages.
 template <int DIR, int CG, int NT, int LCAP, int MAXR = 17>
-__device__ void run(float2* buf, const Plan& p) {
+__device__ void run(float2* buf, const Plan& p, const float2* tw) {
   int ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     switch (R) {
-      case 2: stage<2, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
-      case 3: stage<3, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
-      case 4: stage<4, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
-      case 5: stage<5, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
-      case 7: stage<7, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
-      case 8: stage<8, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw); break;
+      case 2: stage<2, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 3: stage<3, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 4: stage<4, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 5: stage<5, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 7: stage<7, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 8: stage<8, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
       case 11:
-        if constexpr (MAXR >= 11) stage<11, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        if constexpr (MAXR >= 11) stage<11, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
         break;
       case 13:
-        if constexpr (MAXR >= 13) stage<13, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        if constexpr (MAXR >= 13) stage<13, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
         break;
       case 17:
-        if constexpr (MAXR >= 17) stage<17, DIR, CG, NT, LCAP>(buf, p.n, ns, p.tw);
+        if constexpr (MAXR >= 17) stage<17, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
         break;
       default: break;
     }

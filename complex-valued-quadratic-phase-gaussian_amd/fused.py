@@ -116,6 +116,16 @@ class FusedTrainer:
         self.last_pairs = P
         return P <= self.capacity
 
+    def grow(self, headroom=1.25):
+        """Re-size the key buffers for the current parameters (optimiser state and the
+        iteration counter are kept) and re-capture the graph if one was captured."""
+        split = getattr(self, "_split", None)
+        had_graph = self.graph is not None
+        self.graph = None
+        self._size_keys(headroom)
+        if had_graph:
+            self.capture(split=split)
+
     # ------------------------------------------------------------------ stages
     def _s(self):
         return _lib.stream_handle(self.dev)
@@ -205,6 +215,7 @@ class FusedTrainer:
         graphs (stages before ``split``, ``split`` alone, stages after) so the split
         stage can be bracketed by events on the replay stream."""
         torch.cuda.synchronize(self.dev)
+        self._split = split
         stages = self._stages()
         names = [n for n, _ in stages]
         if split is None:
