@@ -35,6 +35,9 @@ constexpr int MAXD = 16;
 #ifndef CVQPG_COL_NT
 #define CVQPG_COL_NT 512
 #endif
+#ifndef CVQPG_WPE
+#define CVQPG_WPE 1
+#endif
 constexpr int ROW_NT = CVQPG_ROW_NT;
 constexpr int ROW_LCAP = 8704;   // longest row transform (complex elements)
 constexpr int ROW_LCAP_SMALL = 4608;
@@ -65,6 +68,7 @@ struct Optics {
 struct BlasmWS {
   float2* tw_r;   // two-level twiddle table of the column length nu
   float2* tw_c;   // two-level twiddle table of the row length nv
+  float2* tw_cs;  // stage tables of a compile-time column plan (8 nu entries)
   float2* X;      // [C][pH][nv]   (forward pass 1 output; adjoint pass 2 output [C][H][nv])
   float2* Y;      // [D][C][oh][nv] (forward pass 2 output; adjoint pass 3 output)
   size_t bytes;
@@ -76,6 +80,7 @@ static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int 
   const size_t nu = 2 * (size_t)pH, nv = 2 * (size_t)pW;
   w.tw_r = cv.take<float2>(fft::twiddle_entries((int)nu));
   w.tw_c = cv.take<float2>(fft::twiddle_entries((int)nv));
+  w.tw_cs = cv.take<float2>(8 * nu);
   const size_t xr = (size_t)(pH > H ? pH : H);
   w.X = cv.take<float2>((size_t)C * xr * nv);
   w.Y = cv.take<float2>((size_t)D * C * oh * nv);
@@ -187,7 +192,7 @@ __device__ __forceinline__ float taper(int t, int total, int inner_start, int in
 // row passes
 // ---------------------------------------------------------------------------------
 // F1: padded/tapered/glued row -> FFT along the 2x row.  grid (pH, C)
-template <int MAXR, int LCAP>
+template <class F>
 __global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ field,
                                                     Optics o, fft::Plan plan,
                                                     float2* __restrict__ X) {
@@ -209,13 +214,13 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ f
     buf[fft::lidx(j)] = u;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
+  F::template exec<1, 1, ROW_NT>(buf, plan, stw);
   float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // F1 for the propagator interface: u = amplitude e^{i phase} on [C, pH, pW].  grid (pH, C)
-template <int MAXR, int LCAP>
+template <class F>
 __global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restrict__ phase,
                                                           const float* __restrict__ amp,
                                                           Optics o, fft::Plan plan,
@@ -236,13 +241,13 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restric
     buf[fft::lidx(j)] = u;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
+  F::template exec<1, 1, ROW_NT>(buf, plan, stw);
   float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // F3: inverse FFT along the row, crop, field and intensity.  grid (oh, D*C)
-template <int MAXR, int LCAP>
+template <class F>
 __global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y, Optics o,
                                                     fft::Plan plan, float2* __restrict__ out,
                                                     float* __restrict__ inten) {
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y
   const float2* src = Y + ((size_t)dc * o.oh + r) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
   __syncthreads();
-  fft::run<-1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
+  F::template exec<-1, 1, ROW_NT>(buf, plan, stw);
   const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
   const float2 cr = o.carrier[dc / o.C][dc % o.C];
   for (int x = threadIdx.x; x < o.ow; x += ROW_NT) {
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y
 
 // A3: adjoint of F3.  g_out = 2 dL/dI * out (times the 1/(nu nv) of the forward), placed in
 // the cropped columns of a zero 2x row, forward FFT.  grid (oh, D*C)
-template <int MAXR, int LCAP>
+template <class F>
 __global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ grad_inten,
                                                     const float2* __restrict__ out, Optics o,
                                                     fft::Plan plan, float2* __restrict__ Z) {
@@ -286,14 +291,14 @@ __global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ gr
     buf[fft::lidx(j)] = g;
   }
   __syncthreads();
-  fft::run<1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
+  F::template exec<1, 1, ROW_NT>(buf, plan, stw);
   float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
 }
 
 // A1: adjoint of F1 on the hologram rows: unnormalised inverse FFT along the row, the
 // hologram columns, adjoint of the glue.  grid (H, C)
-template <int MAXR, int LCAP>
+template <class F>
 __global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G,
                                                     const float2* __restrict__ field, Optics o,
                                                     fft::Plan plan,
@@ -304,7 +309,7 @@ __global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G
   const float2* src = G + ((size_t)c * o.H + y) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
   __syncthreads();
-  fft::run<-1, 1, ROW_NT, LCAP, MAXR>(buf, plan, stw);
+  F::template exec<-1, 1, ROW_NT>(buf, plan, stw);
   const size_t fb = ((size_t)c * o.H + y) * o.W;
   const int c0 = o.zc0 + o.pad_w;
   for (int x = threadIdx.x; x < o.W; x += ROW_NT)
@@ -318,7 +323,7 @@ __device__ __forceinline__ int xcd_block(int hb, int per) { return (hb & 7) * pe
 
 // F2: column FFT, per plane transfer function, inverse column FFT, crop rows.
 // grid (8 * per, C)
-template <int CG, int MAXR>
+template <int CG, class F>
 __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X, Optics o,
                                                     fft::Plan plan_fwd, int per,
                                                     float2* __restrict__ Y) {
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
                                   : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd, stw);
+  F::template exec<1, CG, COL_NT>(buf, plan_fwd, stw);
   float2 S[COL_PER];
   float P[COL_PER];
 #pragma unroll
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
       if (e < total) buf[fft::lidx(e)] = fft::cmul(S[b], tf_value(o, P[b], c, d, e / CG, col0 + e % CG));
     }
     __syncthreads();
-    fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan_fwd, stw);
+    F::template exec<-1, CG, COL_NT>(buf, plan_fwd, stw);
     float2* dst = Y + ((size_t)d * o.C + c) * o.oh * o.nv;
     for (int e = threadIdx.x; e < o.oh * CG; e += COL_NT) {
       const int i = e / CG, g = e % CG;
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
 
 // A2: for every plane, column FFT of the adjoint rows, multiply by conj(H_d), sum; then
 // unnormalised inverse column FFT and keep the hologram rows.  grid (8 * per, C)
-template <int CG, int MAXR>
+template <int CG, class F>
 __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z, Optics o,
                                                     fft::Plan plan, int per,
                                                     float2* __restrict__ G) {
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
       buf[fft::lidx(e)] = (r >= 0 && r < o.oh) ? src[(size_t)r * o.nv + col0 + g] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft::run<1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan, stw);
+    F::template exec<1, CG, COL_NT>(buf, plan, stw);
 #pragma unroll
     for (int b = 0; b < COL_PER; ++b) {
       const int e = threadIdx.x + b * COL_NT;
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
     if (e < total) buf[fft::lidx(e)] = A[b];
   }
   __syncthreads();
-  fft::run<-1, CG, COL_NT, COL_LCAP, MAXR>(buf, plan, stw);
+  F::template exec<-1, CG, COL_NT>(buf, plan, stw);
   float2* dst = G + (size_t)c * o.H * o.nv;
   for (int e = threadIdx.x; e < o.H * CG; e += COL_NT) {
     const int i = e / CG, g = e % CG;
@@ -422,21 +427,472 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
 }
 
 // ---------------------------------------------------------------------------------
+// compile-time pipelines for the transform lengths of the standard workloads
+// ---------------------------------------------------------------------------------
+// The first butterfly stage reads its inputs straight from HBM (or builds them from the
+// raster output) into registers and the last stage writes its outputs straight to HBM,
+// so a transform of L stages makes L - 1 LDS round trips instead of L + 1.  On the 2x
+// grid only positions [N/4, 3N/4) of a padded row or column are non-zero; with a first
+// radix divisible by 4 that is exactly r in [R/4, 3R/4) of every first-stage butterfly,
+// so the zero half is neither loaded nor stored.  The column kernels keep the forward
+// spectrum in the last-stage registers, which are the first-stage inputs of the inverse
+// transform (the inverse runs the radix list reversed).
+
+// RowPlan<NT, Fwd, Inv>: row transform of length Fwd::N on NT threads; ColPlan<CG, NT, WPE,
+// Fwd, Inv>: column transform of CG columns on NT threads compiled for at least WPE waves
+// per SIMD.  Inv runs Fwd's radices reversed (its first stage consumes the forward's
+// last-stage registers); the buffer layouts are the ones a 64-bank conflict model picks
+// for each stage boundary.
+template <class F, class I>
+struct PlanPair {
+  using Fwd = F;
+  using Inv = I;
+  static_assert(F::FIRST == I::LAST && F::LAST == I::FIRST, "inverse must reverse the forward");
+  static constexpr int n = F::n_;
+};
+template <int NT, class F, class I>
+struct RowPlan : PlanPair<F, I> {
+  static constexpr int nt = NT;
+  static constexpr int lds = (F::lds_elems(1) > I::lds_elems(1) ? F::lds_elems(1) : I::lds_elems(1));
+};
+template <int CG, int NT, int WPE, class F, class I>
+struct ColPlan : PlanPair<F, I> {
+  static constexpr int cg = CG, nt = NT, wpe = WPE;
+  static constexpr int lds = (F::lds_elems(CG) > I::lds_elems(CG) ? F::lds_elems(CG) : I::lds_elems(CG));
+};
+using fft::Pipe;
+using fft::Rad;
+using fft::Lays;
+template <int P, int Q>
+using Ly = fft::Lay<P, Q>;
+using Ly0 = fft::Lay<0, 0>;
+
+template <class Pp>
+__device__ __forceinline__ bool in_mid_half(int r) {
+  return r >= Pp::FIRST / 4 && r < 3 * Pp::FIRST / 4;
+}
+
+// F1 (see k_f1_rows) with a compile-time plan.  grid (pH, C)
+template <class P>
+__global__ __launch_bounds__(P::nt) void k_f1_rows_fx(const float2* __restrict__ field,
+                                                      Optics o, fft::Plan plan,
+                                                      float2* __restrict__ X) {
+  constexpr int N = P::n, NT = P::nt;
+  using Fw = typename P::Fwd;
+  using G1 = typename Fw::template G1<1, NT>;
+  using GL = typename Fw::template GL<1, NT>;
+  static_assert(Fw::FIRST % 4 == 0, "first radix must be divisible by 4");
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  const int r = blockIdx.x, c = blockIdx.y;
+  const int sr = min(max(r - o.pad_h, 0), o.H - 1);
+  const float wy = taper(r, o.pH, o.pad_h, o.H);
+  const float2* src = field + ((size_t)c * o.H + sr) * o.W;
+  {
+    float2 v[G1::MAXB][Fw::FIRST];
+#pragma unroll
+    for (int b = 0; b < G1::MAXB; ++b) {
+      if (!G1::live(b)) continue;
+      int j, col;
+      G1::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::FIRST; ++k) {
+        float2 u = make_float2(0.f, 0.f);
+        if (in_mid_half<Fw>(k)) {
+          const int q = j + k * G1::CNT - o.zc0;
+          if ((unsigned)q < (unsigned)o.pW) {
+            const int sc = min(max(q - o.pad_w, 0), o.W - 1);
+            const float w = wy * taper(q, o.pW, o.pad_w, o.W);
+            const float2 h = src[sc];
+            u = glue(make_float2(h.x * w, h.y * w));
+          }
+        }
+        v[b][k] = u;
+      }
+    }
+    Fw::template first<1, 1, NT>(buf, v);
+  }
+  __syncthreads();
+  Fw::template middle<1, 1, NT>(buf, stw, plan.nh);
+  float2 v[GL::MAXB][Fw::LAST];
+  Fw::template last<1, 1, NT>(buf, stw, plan.nh, v);
+  float2* dst = X + ((size_t)c * o.pH + r) * o.nv;
+#pragma unroll
+  for (int b = 0; b < GL::MAXB; ++b) {
+    if (!GL::live(b)) continue;
+    int j, col;
+    GL::item(b, j, col);
+#pragma unroll
+    for (int k = 0; k < Fw::LAST; ++k) dst[j + k * GL::CNT] = v[b][k];
+  }
+}
+
+// F3 (see k_f3_rows) with a compile-time plan.  grid (oh, D*C)
+template <class P>
+__global__ __launch_bounds__(P::nt) void k_f3_rows_fx(const float2* __restrict__ Y, Optics o,
+                                                      fft::Plan plan, float2* __restrict__ out,
+                                                      float* __restrict__ inten) {
+  constexpr int N = P::n, NT = P::nt;
+  using Iv = typename P::Inv;
+  using G1 = typename Iv::template G1<1, NT>;
+  using GL = typename Iv::template GL<1, NT>;
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  const int r = blockIdx.x, dc = blockIdx.y;
+  const float2* src = Y + ((size_t)dc * o.oh + r) * o.nv;
+  {
+    float2 v[G1::MAXB][Iv::FIRST];
+#pragma unroll
+    for (int b = 0; b < G1::MAXB; ++b) {
+      if (!G1::live(b)) continue;
+      int j, col;
+      G1::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Iv::FIRST; ++k) v[b][k] = src[j + k * G1::CNT];
+    }
+    Iv::template first<-1, 1, NT>(buf, v);
+  }
+  __syncthreads();
+  Iv::template middle<-1, 1, NT>(buf, stw, plan.nh);
+  float2 v[GL::MAXB][Iv::LAST];
+  Iv::template last<-1, 1, NT>(buf, stw, plan.nh, v);
+  const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  const float2 cr = o.carrier[dc / o.C][dc % o.C];
+#pragma unroll
+  for (int b = 0; b < GL::MAXB; ++b) {
+    if (!GL::live(b)) continue;
+    int j, col;
+    GL::item(b, j, col);
+#pragma unroll
+    for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {   // the crop lies in [N/4, 3N/4)
+      const int x = j + k * GL::CNT - o.oc0;
+      if ((unsigned)x < (unsigned)o.ow) {
+        const float2 f = make_float2(v[b][k].x * o.norm, v[b][k].y * o.norm);
+        if (out) out[ob + x] = fft::cmul(f, cr);
+        inten[ob + x] = f.x * f.x + f.y * f.y;
+      }
+    }
+  }
+}
+
+// A3 (see k_a3_rows) with a compile-time plan.  grid (oh, D*C)
+template <class P>
+__global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ grad_inten,
+                                                      const float2* __restrict__ out, Optics o,
+                                                      fft::Plan plan, float2* __restrict__ Z) {
+  constexpr int N = P::n, NT = P::nt;
+  using Fw = typename P::Fwd;
+  using G1 = typename Fw::template G1<1, NT>;
+  using GL = typename Fw::template GL<1, NT>;
+  static_assert(Fw::FIRST % 4 == 0, "first radix must be divisible by 4");
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  const int r = blockIdx.x, dc = blockIdx.y;
+  const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
+  const float2 cr = o.carrier[dc / o.C][dc % o.C];
+  const float2 crc = make_float2(cr.x, -cr.y);
+  {
+    float2 v[G1::MAXB][Fw::FIRST];
+#pragma unroll
+    for (int b = 0; b < G1::MAXB; ++b) {
+      if (!G1::live(b)) continue;
+      int j, col;
+      G1::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::FIRST; ++k) {
+        float2 g = make_float2(0.f, 0.f);
+        if (in_mid_half<Fw>(k)) {
+          const int x = j + k * G1::CNT - o.oc0;
+          if ((unsigned)x < (unsigned)o.ow) {
+            const float sc = 2.0f * grad_inten[ob + x] * o.norm;
+            const float2 f = fft::cmul(out[ob + x], crc);
+            g = make_float2(sc * f.x, sc * f.y);
+          }
+        }
+        v[b][k] = g;
+      }
+    }
+    Fw::template first<1, 1, NT>(buf, v);
+  }
+  __syncthreads();
+  Fw::template middle<1, 1, NT>(buf, stw, plan.nh);
+  float2 v[GL::MAXB][Fw::LAST];
+  Fw::template last<1, 1, NT>(buf, stw, plan.nh, v);
+  float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
+#pragma unroll
+  for (int b = 0; b < GL::MAXB; ++b) {
+    if (!GL::live(b)) continue;
+    int j, col;
+    GL::item(b, j, col);
+#pragma unroll
+    for (int k = 0; k < Fw::LAST; ++k) dst[j + k * GL::CNT] = v[b][k];
+  }
+}
+
+// A1 (see k_a1_rows) with a compile-time plan.  grid (H, C)
+template <class P>
+__global__ __launch_bounds__(P::nt) void k_a1_rows_fx(const float2* __restrict__ G,
+                                                      const float2* __restrict__ field, Optics o,
+                                                      fft::Plan plan,
+                                                      float2* __restrict__ grad_field) {
+  constexpr int N = P::n, NT = P::nt;
+  using Iv = typename P::Inv;
+  using G1 = typename Iv::template G1<1, NT>;
+  using GL = typename Iv::template GL<1, NT>;
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  const int y = blockIdx.x, c = blockIdx.y;
+  const float2* src = G + ((size_t)c * o.H + y) * o.nv;
+  {
+    float2 v[G1::MAXB][Iv::FIRST];
+#pragma unroll
+    for (int b = 0; b < G1::MAXB; ++b) {
+      if (!G1::live(b)) continue;
+      int j, col;
+      G1::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Iv::FIRST; ++k) v[b][k] = src[j + k * G1::CNT];
+    }
+    Iv::template first<-1, 1, NT>(buf, v);
+  }
+  __syncthreads();
+  Iv::template middle<-1, 1, NT>(buf, stw, plan.nh);
+  float2 v[GL::MAXB][Iv::LAST];
+  Iv::template last<-1, 1, NT>(buf, stw, plan.nh, v);
+  const size_t fb = ((size_t)c * o.H + y) * o.W;
+  const int c0 = o.zc0 + o.pad_w;
+#pragma unroll
+  for (int b = 0; b < GL::MAXB; ++b) {
+    if (!GL::live(b)) continue;
+    int j, col;
+    GL::item(b, j, col);
+#pragma unroll
+    for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {   // the hologram lies in [N/4, 3N/4)
+      const int x = j + k * GL::CNT - c0;
+      if ((unsigned)x < (unsigned)o.W) grad_field[fb + x] = glue_adjoint(field[fb + x], v[b][k]);
+    }
+  }
+}
+
+// LDS of a column kernel: data, two-level table, then (16-byte aligned) stage tables
+template <class P>
+__host__ __device__ constexpr int col_lds_head() {
+  return (P::lds + fft::twiddle_entries(P::n) + 1) & ~1;
+}
+
+// global stage tables of a column plan: the forward pipe's, then the inverse pipe's
+template <class P>
+__global__ void k_pipe_tables(float2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P::Fwd::TABLE) P::Fwd::table_entry(out, i);
+  else if (i < P::Fwd::TABLE + P::Inv::TABLE) P::Inv::table_entry(out + P::Fwd::TABLE, i - P::Fwd::TABLE);
+}
+
+// F2 (see k_f2_cols) with a compile-time plan.  The per-plane inverse transforms read
+// their twiddles from the inverse pipe's stage tables, staged in LDS once per
+// workgroup; only the rows of the inverse that can fall in the crop ([N/4, 3N/4)) are
+// formed.  grid (8 * per, C)
+template <class P>
+__global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __restrict__ X, Optics o,
+                                                              fft::Plan plan,
+                                                              const float2* __restrict__ tabs,
+                                                              int per, float2* __restrict__ Y) {
+  constexpr int N = P::n, NT = P::nt, CG = P::cg;
+  using Fw = typename P::Fwd;
+  using Iv = typename P::Inv;
+  using F1 = typename Fw::template G1<CG, NT>;
+  using FL = typename Fw::template GL<CG, NT>;   // == first stage of the inverse
+  using IL = typename Iv::template GL<CG, NT>;
+  static_assert(Fw::FIRST % 4 == 0 && Iv::LAST % 4 == 0, "first radix must be divisible by 4");
+  static_assert(Fw::LAST == Iv::FIRST, "inverse must start with the forward's last radix");
+  extern __shared__ float2 buf[];
+  const int lb = xcd_block(blockIdx.x, per);
+  const int col0 = lb * CG;
+  if (col0 >= o.nv) return;
+  const int c = blockIdx.y;
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  float2* tt = buf + col_lds_head<P>();
+  for (int i = threadIdx.x; i < Iv::TABLE; i += NT) tt[i] = tabs[Fw::TABLE + i];
+  {
+    float2 v[F1::MAXB][Fw::FIRST];
+#pragma unroll
+    for (int b = 0; b < F1::MAXB; ++b) {
+      if (!F1::live(b)) continue;
+      int j, col;
+      F1::item(b, j, col);
+      const float2* xb = X + ((ptrdiff_t)c * o.pH + j - o.zr0) * o.nv + col0 + col;
+#pragma unroll
+      for (int k = 0; k < Fw::FIRST; ++k) {
+        float2 u = make_float2(0.f, 0.f);
+        if (in_mid_half<Fw>(k)) {
+          const int rr = j + k * F1::CNT - o.zr0;
+          if ((unsigned)rr < (unsigned)o.pH) u = xb[(size_t)(k * F1::CNT) * o.nv];
+        }
+        v[b][k] = u;
+      }
+    }
+    Fw::template first<1, CG, NT>(buf, v);
+  }
+  __syncthreads();
+  Fw::template middle<1, CG, NT>(buf, stw, plan.nh);
+  float2 S[FL::MAXB][Fw::LAST];
+  float Pg[FL::MAXB][Fw::LAST];
+  Fw::template last<1, CG, NT>(buf, stw, plan.nh, S);
+#pragma unroll
+  for (int b = 0; b < FL::MAXB; ++b) {
+    if (!FL::live(b)) continue;
+    int j, col;
+    FL::item(b, j, col);
+#pragma unroll
+    for (int k = 0; k < Fw::LAST; ++k) Pg[b][k] = tf_point(o, c, j + k * FL::CNT, col0 + col);
+  }
+  for (int d = 0; d < o.D; ++d) {
+    __syncthreads();   // the previous reads of buf are complete
+    {
+      float2 v[FL::MAXB][Fw::LAST];
+#pragma unroll
+      for (int b = 0; b < FL::MAXB; ++b) {
+        if (!FL::live(b)) continue;
+        int j, col;
+        FL::item(b, j, col);
+#pragma unroll
+        for (int k = 0; k < Fw::LAST; ++k)
+          v[b][k] = fft::cmul(S[b][k], tf_value(o, Pg[b][k], c, d, j + k * FL::CNT, col0 + col));
+      }
+      Iv::template first<-1, CG, NT>(buf, v);
+    }
+    __syncthreads();
+    Iv::template middle<-1, CG, NT, true>(buf, stw, plan.nh, tt);
+    float2 u[IL::MAXB][Iv::LAST];
+    Iv::template last<-1, CG, NT, true>(buf, stw, plan.nh, u, tt);
+    float2* dst = Y + ((size_t)d * o.C + c) * o.oh * o.nv + col0;
+#pragma unroll
+    for (int b = 0; b < IL::MAXB; ++b) {
+      if (!IL::live(b)) continue;
+      int j, col;
+      IL::item(b, j, col);
+      float2* db = dst + (ptrdiff_t)(j - o.or0) * o.nv + col;
+#pragma unroll
+      for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
+        const int rr = j + k * IL::CNT - o.or0;
+        if ((unsigned)rr < (unsigned)o.oh) db[(size_t)(k * IL::CNT) * o.nv] = u[b][k];
+      }
+    }
+  }
+}
+
+// A2 (see k_a2_cols) with a compile-time plan.  The per-plane forward transforms read
+// their twiddles from the forward pipe's stage tables staged in LDS.  grid (8 * per, C)
+template <class P>
+__global__ __launch_bounds__(P::nt, P::wpe) void k_a2_cols_fx(const float2* __restrict__ Z, Optics o,
+                                                              fft::Plan plan,
+                                                              const float2* __restrict__ tabs,
+                                                              int per, float2* __restrict__ G) {
+  constexpr int N = P::n, NT = P::nt, CG = P::cg;
+  using Fw = typename P::Fwd;
+  using Iv = typename P::Inv;
+  using F1 = typename Fw::template G1<CG, NT>;
+  using FL = typename Fw::template GL<CG, NT>;
+  using IL = typename Iv::template GL<CG, NT>;
+  static_assert(Fw::FIRST % 4 == 0 && Iv::LAST % 4 == 0, "first radix must be divisible by 4");
+  static_assert(Fw::LAST == Iv::FIRST, "inverse must start with the forward's last radix");
+  extern __shared__ float2 buf[];
+  const int lb = xcd_block(blockIdx.x, per);
+  const int col0 = lb * CG;
+  if (col0 >= o.nv) return;
+  const int c = blockIdx.y;
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  float2* tt = buf + col_lds_head<P>();
+  for (int i = threadIdx.x; i < Fw::TABLE; i += NT) tt[i] = tabs[i];
+  float2 A[FL::MAXB][Fw::LAST];
+  float Pg[FL::MAXB][Fw::LAST];
+#pragma unroll
+  for (int b = 0; b < FL::MAXB; ++b) {
+    int j, col;
+    FL::item(b, j, col);
+#pragma unroll
+    for (int k = 0; k < Fw::LAST; ++k) {
+      A[b][k] = make_float2(0.f, 0.f);
+      Pg[b][k] = FL::live(b) ? tf_point(o, c, j + k * FL::CNT, col0 + col) : -1.0f;
+    }
+  }
+  for (int d = 0; d < o.D; ++d) {
+    if (d > 0) __syncthreads();   // the previous plane's reads of buf are complete
+    const float2* src = Z + ((size_t)d * o.C + c) * o.oh * o.nv + col0;
+    {
+      float2 v[F1::MAXB][Fw::FIRST];
+#pragma unroll
+      for (int b = 0; b < F1::MAXB; ++b) {
+        if (!F1::live(b)) continue;
+        int j, col;
+        F1::item(b, j, col);
+        const float2* sb = src + (ptrdiff_t)(j - o.or0) * o.nv + col;
+#pragma unroll
+        for (int k = 0; k < Fw::FIRST; ++k) {
+          float2 u = make_float2(0.f, 0.f);
+          if (in_mid_half<Fw>(k)) {
+            const int rr = j + k * F1::CNT - o.or0;
+            if ((unsigned)rr < (unsigned)o.oh) u = sb[(size_t)(k * F1::CNT) * o.nv];
+          }
+          v[b][k] = u;
+        }
+      }
+      Fw::template first<1, CG, NT>(buf, v);
+    }
+    __syncthreads();
+    Fw::template middle<1, CG, NT, true>(buf, stw, plan.nh, tt);
+    float2 v[FL::MAXB][Fw::LAST];
+    Fw::template last<1, CG, NT, true>(buf, stw, plan.nh, v, tt);
+#pragma unroll
+    for (int b = 0; b < FL::MAXB; ++b) {
+      if (!FL::live(b)) continue;
+      int j, col;
+      FL::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k) {
+        const float2 h = tf_value(o, Pg[b][k], c, d, j + k * FL::CNT, col0 + col);
+        A[b][k].x += v[b][k].x * h.x + v[b][k].y * h.y;   // v * conj(h)
+        A[b][k].y += v[b][k].y * h.x - v[b][k].x * h.y;
+      }
+    }
+  }
+  __syncthreads();
+  Iv::template first<-1, CG, NT>(buf, A);
+  __syncthreads();
+  Iv::template middle<-1, CG, NT>(buf, stw, plan.nh);
+  float2 u[IL::MAXB][Iv::LAST];
+  Iv::template last<-1, CG, NT>(buf, stw, plan.nh, u);
+  float2* dst = G + (size_t)c * o.H * o.nv + col0;
+#pragma unroll
+  for (int b = 0; b < IL::MAXB; ++b) {
+    if (!IL::live(b)) continue;
+    int j, col;
+    IL::item(b, j, col);
+    float2* db = dst + (ptrdiff_t)(j - o.gr0) * o.nv + col;
+#pragma unroll
+    for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
+      const int rr = j + k * IL::CNT - o.gr0;
+      if ((unsigned)rr < (unsigned)o.H) db[(size_t)(k * IL::CNT) * o.nv] = u[b][k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------------
-// row kernels are instantiated for radices <= 8 (lean) and <= 17
-// row kernels are instantiated per radix class (<= 8, <= 17) and length class
-// (<= 4608, <= 8704) so the per-thread butterfly registers match the transform
-#define ROW_LAUNCH(K, PLAN, GRID, SH, STREAM, ...)                                   \
-  do {                                                                               \
-    const bool small_ = (PLAN).n <= ROW_LCAP_SMALL;                                  \
-    if (fft::max_radix(PLAN) <= 8) {                                                 \
-      if (small_) launch_row(K<8, ROW_LCAP_SMALL>, GRID, SH, STREAM, __VA_ARGS__);   \
-      else launch_row(K<8, ROW_LCAP>, GRID, SH, STREAM, __VA_ARGS__);                \
-    } else {                                                                         \
-      if (small_) launch_row(K<17, ROW_LCAP_SMALL>, GRID, SH, STREAM, __VA_ARGS__);  \
-      else launch_row(K<17, ROW_LCAP>, GRID, SH, STREAM, __VA_ARGS__);               \
-    }                                                                                \
+// Row kernels on runtime plans, instantiated per radix class (<= 8, <= 17) and length
+// class (<= 4608, <= 8704) so the per-thread butterfly registers match the transform.
+#define ROW_LAUNCH(K, PLAN, GRID, SH, STREAM, ...)                                          \
+  do {                                                                                      \
+    const int n_ = (PLAN).n;                                                                \
+    const bool small_ = n_ <= ROW_LCAP_SMALL;                                               \
+    if (fft::max_radix(PLAN) <= 8) {                                                        \
+      if (small_) launch_row(K<fft::Runtime<8, ROW_LCAP_SMALL>>, GRID, SH, STREAM, __VA_ARGS__); \
+      else launch_row(K<fft::Runtime<8, ROW_LCAP>>, GRID, SH, STREAM, __VA_ARGS__);         \
+    } else {                                                                                \
+      if (small_) launch_row(K<fft::Runtime<17, ROW_LCAP_SMALL>>, GRID, SH, STREAM, __VA_ARGS__); \
+      else launch_row(K<fft::Runtime<17, ROW_LCAP>>, GRID, SH, STREAM, __VA_ARGS__);        \
+    }                                                                                       \
   } while (0)
 
 // dynamic LDS above 64 KB must be enabled per kernel
@@ -455,6 +911,41 @@ static void launch_row(Kern k, dim3 grid, size_t sh, hipStream_t s, Args... args
 
 static size_t row_lds_bytes(const Optics& o) {
   return (size_t)(fft::lds_elems(o.nv) + fft::twiddle_entries(o.nv)) * sizeof(float2);
+}
+
+// compile-time row plans of the standard workloads (4480: 1080p, 8320: 2160p, 1152: 256^2)
+using RowP4480 = RowPlan<320, Pipe<4480, Rad<16, 8, 5, 7>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
+                         Pipe<4480, Rad<7, 5, 8, 16>, Lays<Ly0, Ly<32, 4>, Ly0>>>;
+using RowP8320 = RowPlan<640, Pipe<8320, Rad<16, 8, 5, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
+                         Pipe<8320, Rad<13, 5, 8, 16>, Lays<Ly0, Ly<32, 4>, Ly0>>>;
+using RowP1152 = RowPlan<128, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 1>, Ly<32, 4>>>,
+                         Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<64, 1>>>>;
+// compile-time column plans (2816: 1080p, 1152: 256^2, 4992: 2160p)
+using ColP2816 = ColPlan<2, 512, CVQPG_WPE, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 2>, Ly0>>,
+                         Pipe<2816, Rad<11, 16, 16>, Lays<Ly0, Ly<32, 2>>>>;
+using ColP1152 = ColPlan<4, 512, 2, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 2>, Ly0>>,
+                         Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<128, 2>>>>;
+using ColP4992 = ColPlan<1, 384, 2, Pipe<4992, Rad<16, 8, 3, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
+                         Pipe<4992, Rad<13, 3, 8, 16>, Lays<Ly0, Ly0, Ly0>>>;
+
+template <class P>
+static size_t row_fx_lds_bytes() {
+  return (size_t)(P::lds + fft::twiddle_entries(P::n)) * sizeof(float2);
+}
+template <class Fn>
+static bool with_row_plan(int n, Fn&& fn) {
+  switch (n) {
+    case 4480: fn(RowP4480{}); return true;
+    case 8320: fn(RowP8320{}); return true;
+    case 1152: fn(RowP1152{}); return true;
+    default: return false;
+  }
+}
+
+template <class P, class Kern, class... Args>
+static void launch_fx(Kern k, dim3 grid, size_t sh, hipStream_t s, Args... args) {
+  allow_lds(k, sh);
+  hipLaunchKernelGGL(k, grid, dim3(P::nt), sh, s, args...);
 }
 
 static int pick_cg(int nu, int nv) {
@@ -489,6 +980,12 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   o.gr0 = o.zr0 + o.pad_h;
   o.cg = pick_cg(o.nu, o.nv);
   CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length", pH);
+  // the pruned transforms rely on every non-zero input and every kept output lying in
+  // the centre half [n/4, 3n/4) of the 2x grid
+  CVQPG_CHECK_ARG(o.zr0 == o.nu / 4 && o.zc0 == o.nv / 4 && o.or0 >= o.nu / 4 &&
+                      o.or0 + o.oh <= 3 * (o.nu / 4) && o.oc0 >= o.nv / 4 &&
+                      o.oc0 + o.ow <= 3 * (o.nv / 4),
+                  "crop window outside the centre half of the padded grid");
   o.norm = (float)(1.0 / ((double)o.nu * (double)o.nv));
   const double X = pitch * o.nu, Y = pitch * o.nv;
   double fr0, frs, fc0, fcs;
@@ -547,32 +1044,74 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   return 0;
 }
 
+// Column kernels: compile-time plans for the column lengths of the standard workloads
+// (with the column-group width pick_cg chooses for them), runtime plans otherwise.
+template <int CG, class F>
+static void launch_f2_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  const int nb = o.nv / CG, per = ceil_div(nb, 8);
+  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
+  allow_lds(k_f2_cols<CG, F>, sh);
+  hipLaunchKernelGGL((k_f2_cols<CG, F>), dim3(8 * per, o.C), dim3(COL_NT), sh, s, w.X, o, pr,
+                     per, w.Y);
+}
+
+template <int CG, class F>
+static void launch_a2_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  const int nb = o.nv / CG, per = ceil_div(nb, 8);
+  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
+  allow_lds(k_a2_cols<CG, F>, sh);
+  hipLaunchKernelGGL((k_a2_cols<CG, F>), dim3(8 * per, o.C), dim3(COL_NT), sh, s, w.Y, o, pr,
+                     per, w.X);
+}
+
+template <int CG, bool ADJ, class F>
+static void launch_cols_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  if constexpr (ADJ) launch_a2_k<CG, F>(w, o, pr, s);
+  else launch_f2_k<CG, F>(w, o, pr, s);
+}
+
+template <class P, bool ADJ>
+static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  static_assert(P::Fwd::TABLE + P::Inv::TABLE <= 8 * P::n, "stage tables exceed their region");
+  constexpr int ne = P::Fwd::TABLE + P::Inv::TABLE;
+  hipLaunchKernelGGL(k_pipe_tables<P>, dim3(ceil_div(ne, 256)), dim3(256), 0, s, w.tw_cs);
+  const int nb = o.nv / P::cg, per = ceil_div(nb, 8);
+  const int tab = ADJ ? P::Fwd::TABLE : P::Inv::TABLE;
+  const size_t sh = (size_t)(col_lds_head<P>() + tab) * sizeof(float2);
+  if constexpr (ADJ)
+    launch_fx<P>(k_a2_cols_fx<P>, dim3(8 * per, o.C), sh, s, (const float2*)w.Y, o, pr,
+                 (const float2*)w.tw_cs, per, w.X);
+  else
+    launch_fx<P>(k_f2_cols_fx<P>, dim3(8 * per, o.C), sh, s, (const float2*)w.X, o, pr,
+                 (const float2*)w.tw_cs, per, w.Y);
+}
+
+template <int CG, bool ADJ>
+static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+  // compile-time column plans of the standard workloads (2816: 1080p, 4992: 2160p, 1152: 256^2)
+  if constexpr (CG == 2) {
+    if (o.nu == 2816) return launch_cols_fx<ColP2816, ADJ>(w, o, pr, s);
+  }
+  if constexpr (CG == 4) {
+    if (o.nu == 1152) return launch_cols_fx<ColP1152, ADJ>(w, o, pr, s);
+  }
+  if constexpr (CG == 1) {
+    if (o.nu == 4992) return launch_cols_fx<ColP4992, ADJ>(w, o, pr, s);
+  }
+  const int mr = fft::max_radix(pr);
+  if (mr <= 8) launch_cols_k<CG, ADJ, fft::Runtime<8, COL_LCAP>>(w, o, pr, s);
+  else if (mr <= 13) launch_cols_k<CG, ADJ, fft::Runtime<13, COL_LCAP>>(w, o, pr, s);
+  else launch_cols_k<CG, ADJ, fft::Runtime<17, COL_LCAP>>(w, o, pr, s);
+}
+
 template <int CG>
 static void launch_f2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
-  const int nb = o.nv / CG, per = ceil_div(nb, 8);
-  const dim3 g(8 * per, o.C), b(COL_NT);
-  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
-  const int mr = fft::max_radix(pr);
-  if (mr <= 8)
-    { allow_lds(k_f2_cols<CG, 8>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 8>), g, b, sh, s, w.X, o, pr, per, w.Y); }
-  else if (mr <= 13)
-    { allow_lds(k_f2_cols<CG, 13>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 13>), g, b, sh, s, w.X, o, pr, per, w.Y); }
-  else
-    { allow_lds(k_f2_cols<CG, 17>, sh); hipLaunchKernelGGL((k_f2_cols<CG, 17>), g, b, sh, s, w.X, o, pr, per, w.Y); }
+  launch_cols<CG, false>(w, o, pr, s);
 }
 
 template <int CG>
 static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
-  const int nb = o.nv / CG, per = ceil_div(nb, 8);
-  const dim3 g(8 * per, o.C), b(COL_NT);
-  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
-  const int mr = fft::max_radix(pr);
-  if (mr <= 8)
-    { allow_lds(k_a2_cols<CG, 8>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 8>), g, b, sh, s, w.Y, o, pr, per, w.X); }
-  else if (mr <= 13)
-    { allow_lds(k_a2_cols<CG, 13>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 13>), g, b, sh, s, w.Y, o, pr, per, w.X); }
-  else
-    { allow_lds(k_a2_cols<CG, 17>, sh); hipLaunchKernelGGL((k_a2_cols<CG, 17>), g, b, sh, s, w.Y, o, pr, per, w.X); }
+  launch_cols<CG, true>(w, o, pr, s);
 }
 
 static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc,
@@ -595,8 +1134,13 @@ static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan&
     case 2: launch_f2<2>(w, o, pr, s); break;
     default: launch_f2<1>(w, o, pr, s); break;
   }
-  ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, w.Y, o, pc,
-                     reinterpret_cast<float2*>(out_field), intensity);
+  float2* of = reinterpret_cast<float2*>(out_field);
+  const bool fx = with_row_plan(o.nv, [&](auto pl) {
+    using P = decltype(pl);
+    launch_fx<P>(k_f3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.Y,
+                 o, pc, of, intensity);
+  });
+  if (!fx) ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, w.Y, o, pc, of, intensity);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }
@@ -629,8 +1173,12 @@ extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height,
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s,
-                     reinterpret_cast<const float2*>(field), o, pc, w.X);
+  const float2* fl = reinterpret_cast<const float2*>(field);
+  const bool fx = with_row_plan(o.nv, [&](auto pl) {
+    using P = decltype(pl);
+    launch_fx<P>(k_f1_rows_fx<P>, dim3(o.pH, o.C), row_fx_lds_bytes<P>(), s, fl, o, pc, w.X);
+  });
+  if (!fx) ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, fl, o, pc, w.X);
   CVQPG_LAUNCH_CHECK();
   return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
 }
@@ -653,17 +1201,27 @@ extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intens
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity,
-                     reinterpret_cast<const float2*>(out_field), o, pc, w.Y);
+  const float2* of = reinterpret_cast<const float2*>(out_field);
+  const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
+    using P = decltype(pl);
+    launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, grad_intensity, of,
+                 o, pc, w.Y);
+  });
+  if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
   switch (o.cg) {
     case 8: launch_a2<8>(w, o, pr, s); break;
     case 4: launch_a2<4>(w, o, pr, s); break;
     case 2: launch_a2<2>(w, o, pr, s); break;
     default: launch_a2<1>(w, o, pr, s); break;
   }
-  ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s,
-                     w.X, reinterpret_cast<const float2*>(field), o, pc,
-                     reinterpret_cast<float2*>(grad_field));
+  const float2* fl = reinterpret_cast<const float2*>(field);
+  float2* gf = reinterpret_cast<float2*>(grad_field);
+  const bool fx1 = with_row_plan(o.nv, [&](auto pl) {
+    using P = decltype(pl);
+    launch_fx<P>(k_a1_rows_fx<P>, dim3(o.H, o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.X, fl, o,
+                 pc, gf);
+  });
+  if (!fx1) ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s, w.X, fl, o, pc, gf);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }

@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <tuple>
+
 namespace cvqpg {
 namespace fft {
 
@@ -33,7 +35,7 @@ struct Plan {
 };
 
 constexpr int TW_LO = 64;
-inline int twiddle_entries(int n) { return (n + TW_LO - 1) / TW_LO + TW_LO; }
+__host__ __device__ inline int twiddle_entries(int n) { return (n + TW_LO - 1) / TW_LO + TW_LO; }
 
 // LDS element index with one padding slot every 16 complex elements, so the strided
 // accesses of the early Stockham stages spread over the banks.
@@ -133,6 +135,46 @@ __device__ __forceinline__ void dft8(float2* v) {
   for (int k = 0; k < 4; ++k) {
     v[k] = cadd(e[k], o[k]);
     v[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* v) {
+  // 16 = 4 x 4: DFT4 over n1 for each n2 (x[4 n1 + n2]), twiddle W16^{n2 k1}, DFT4 over n2
+  constexpr float C1 = 0.92387953251128676f, S1 = 0.38268343236508977f;
+  constexpr float S2 = 0.70710678118654752f;
+  float2 a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    a[n2][0] = v[n2];
+    a[n2][1] = v[4 + n2];
+    a[n2][2] = v[8 + n2];
+    a[n2][3] = v[12 + n2];
+    dft4<DIR>(a[n2]);
+  }
+  // W16^m for m = n2 k1 in {1, 2, 3, 4, 6, 9}; forward uses e^{-i}
+  auto tw = [](float2 x, float c, float s) {
+    // x * (c - i s) forward, x * (c + i s) inverse
+    return DIR > 0 ? make_float2(x.x * c + x.y * s, x.y * c - x.x * s)
+                   : make_float2(x.x * c - x.y * s, x.y * c + x.x * s);
+  };
+  a[1][1] = tw(a[1][1], C1, S1);
+  a[1][2] = tw(a[1][2], S2, S2);
+  a[1][3] = tw(a[1][3], S1, C1);
+  a[2][1] = tw(a[2][1], S2, S2);
+  a[2][2] = rot90<DIR>(a[2][2]);
+  a[2][3] = tw(a[2][3], -S2, S2);
+  a[3][1] = tw(a[3][1], S1, C1);
+  a[3][2] = tw(a[3][2], -S2, S2);
+  a[3][3] = tw(a[3][3], -C1, -S1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4<DIR>(b);
+    v[k1] = b[0];
+    v[k1 + 4] = b[1];
+    v[k1 + 8] = b[2];
+    v[k1 + 12] = b[3];
   }
 }
 
@@ -252,6 +294,8 @@ __device__ __forceinline__ void dft(float2* v) {
     dft5<DIR>(v);
   } else if constexpr (R == 8) {
     dft8<DIR>(v);
+  } else if constexpr (R == 16) {
+    dft16<DIR>(v);
   } else {
     dft_odd<R, DIR>(v);
   }
@@ -308,7 +352,7 @@ __device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* 
 // the radices compiled into this instance: kernels whose plans only use radices <= 8
 // instantiate MAXR = 8 and do not pay the registers of the large direct-DFT stages.
 template <int DIR, int CG, int NT, int LCAP, int MAXR = 17>
-__device__ void run(float2* buf, const Plan& p, const float2* tw) {
+__device__ __forceinline__ void run(float2* buf, const Plan& p, const float2* tw) {
   int ns = 1;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
@@ -339,6 +383,238 @@ inline int max_radix(const Plan& p) {
   for (int s = 0; s < p.nstages; ++s) m = p.radix[s] > m ? p.radix[s] : m;
   return m;
 }
+
+// ------------------------------------------------------------------------------
+// Compile-time plans: the stage sequence, the stride ns and every per-thread butterfly
+// count are constants, so the index arithmetic folds to shifts/multiplies and each stage
+// holds exactly the registers it needs.
+// ------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------
+// compile-time transforms
+// ---------------------------------------------------------------------------------
+// Geo<N, R, CG, NT>: work split of one radix-R stage of length N over CG interleaved
+// sequences and NT threads; butterfly b of a thread is w = threadIdx.x + b NT, which
+// covers column w mod CG and butterfly j = w / CG.
+template <int N, int R, int CG, int NT>
+struct Geo {
+  static constexpr int CNT = N / R;
+  static constexpr int TOTAL = CNT * CG;
+  static constexpr int MAXB = (TOTAL + NT - 1) / NT;
+  __device__ __forceinline__ static bool live(int b) {
+    return TOTAL % NT == 0 || (int)threadIdx.x + b * NT < TOTAL;
+  }
+  __device__ __forceinline__ static void item(int b, int& j, int& col) {
+    const int w = threadIdx.x + b * NT;
+    col = w % CG;
+    j = w / CG;
+  }
+};
+
+// LDS layout of one inter-stage buffer: Q padding slots after every P elements (P a
+// power of two; P = 0: unpadded).  The LDS has 64 banks of 4 bytes, so a wave's 64
+// eight-byte accesses are conflict-free when they cover every bank exactly twice.
+// Contiguous accesses want no padding, the strided writes of the early Stockham stages
+// want some; each buffer of a compile-time pipe gets the layout that suits its writer
+// and its reader.
+template <int P, int Q>
+struct Lay {
+  static_assert(P == 0 || (P & (P - 1)) == 0, "P must be a power of two");
+  static constexpr int P_ = P, Q_ = Q;
+  __device__ __forceinline__ static int at(int i) {
+    if constexpr (P == 0) return i;
+    else return i + (i / P) * Q;
+  }
+  // slot of element base + r STRIDE: when STRIDE is a multiple of P the padding of every
+  // element follows from the base's, so the accesses of a butterfly share one address
+  // register and differ by immediate offsets
+  template <int STRIDE>
+  __device__ __forceinline__ static int slot(int base, int r) {
+    if constexpr (P == 0) return base + r * STRIDE;
+    else if constexpr (STRIDE % P == 0) return at(base) + r * (STRIDE + STRIDE / P * Q);
+    else return at(base + r * STRIDE);
+  }
+  static constexpr int elems(int n) { return P == 0 ? n : n + (n / P) * Q + Q; }
+};
+
+// Inputs of stage (R, NS) from LDS buffer layout LI, twiddled and transformed, into
+// registers.  For the first stage (NS = 1) and the last (NS = N / R) of a transform,
+// element r of butterfly j is sequence position j + r N / R.  Twiddles come from the
+// stage table tt (TAB) or from the two-level table tw as powers of W^k.  Stage (R, NS)
+// with NS > 1 stores W_{NS R}^{r k} at (r - 1) NS + k, so the lanes of a wave
+// (consecutive k) read consecutive words.
+template <int R, int NS, int N, int DIR, int CG, int NT, bool TAB, class LI>
+__device__ __forceinline__ void sread(const float2* buf, const float2* tw, int nh,
+                                      const float2* tt, float2 (&v)[Geo<N, R, CG, NT>::MAXB][R]) {
+  using G = Geo<N, R, CG, NT>;
+  constexpr int TSTEP = N / (NS * R);
+#pragma unroll
+  for (int b = 0; b < G::MAXB; ++b) {
+    if (!G::live(b)) continue;
+    int j, col;
+    G::item(b, j, col);
+    const int base = j * CG + col;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b][r] = buf[LI::template slot<G::CNT * CG>(base, r)];
+    if constexpr (NS > 1) {
+      const int k = j % NS;
+      if constexpr (TAB) {
+        const float2* t = tt + k;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float2 w = t[(r - 1) * NS];
+          v[b][r] = cmul(v[b][r], DIR > 0 ? w : make_float2(w.x, -w.y));
+        }
+      } else {
+        const float2 w1 = twid<DIR>(tw, nh, k * TSTEP);
+        float2 wr = w1;
+        v[b][1] = cmul(v[b][1], w1);
+#pragma unroll
+        for (int r = 2; r < R; ++r) {
+          wr = cmul(wr, w1);
+          v[b][r] = cmul(v[b][r], wr);
+        }
+      }
+    }
+    dft<R, DIR>(v[b]);
+  }
+}
+
+// Outputs of stage (R, NS) from registers to their Stockham positions in LDS buffer
+// layout LO.
+template <int R, int NS, int N, int CG, int NT, class LO>
+__device__ __forceinline__ void swrite(float2* buf, const float2 (&v)[Geo<N, R, CG, NT>::MAXB][R]) {
+  using G = Geo<N, R, CG, NT>;
+#pragma unroll
+  for (int b = 0; b < G::MAXB; ++b) {
+    if (!G::live(b)) continue;
+    int j, col;
+    G::item(b, j, col);
+    if constexpr (NS == 1 && LO::P_ > 0 && (R * CG) % LO::P_ == 0 && LO::P_ % CG == 0) {
+      // j R CG is a multiple of P and col + r CG stays in the P-block of r CG
+      const int a = LO::at(j * R * CG) + col;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[a + r * CG + (r * CG / LO::P_) * LO::Q_] = v[b][r];
+    } else {
+      const int dst = ((j / NS) * NS * R + j % NS) * CG + col;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[LO::template slot<NS * CG>(dst, r)] = v[b][r];
+    }
+  }
+}
+
+// One LDS-to-LDS stage; all reads finish before the barrier, all writes before the next.
+template <int R, int NS, int N, int DIR, int CG, int NT, bool TAB, class LI, class LO>
+__device__ __forceinline__ void cstage(float2* buf, const float2* tw, int nh, const float2* tt) {
+  float2 v[Geo<N, R, CG, NT>::MAXB][R];
+  sread<R, NS, N, DIR, CG, NT, TAB, LI>(buf, tw, nh, tt, v);
+  __syncthreads();
+  swrite<R, NS, N, CG, NT, LO>(buf, v);
+  __syncthreads();
+}
+
+template <int... Rs>
+struct Rad {};
+template <class... Ls>
+struct Lays {};
+
+// Pipe<N, Rad<R1, ..., RL>, Lays<B1, ..., B(L-1)>>: a transform of length N = prod(R)
+// whose first stage takes its inputs from registers (NS = 1, no twiddles) and whose last
+// stage leaves its outputs in registers, both indexed as position j + r N / R.  Buffer
+// Bs holds the output of stage s.  With TAB the middle and last stages read their
+// twiddles from the concatenated stage tables at tt (TABLE entries, stage order).
+template <int N, class RL, class LL>
+struct Pipe;
+template <int N, int... Rs, class... Ls>
+struct Pipe<N, Rad<Rs...>, Lays<Ls...>> {
+  static constexpr int L = sizeof...(Rs);
+  static_assert(L >= 2, "a pipe has at least two stages");
+  static_assert(sizeof...(Ls) == L - 1, "one buffer layout per stage boundary");
+  static constexpr int RR[L] = {Rs...};
+  static constexpr int prod(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= RR[i];
+    return p;
+  }
+  static_assert(prod(L) == N, "radices must multiply to N");
+  static constexpr int toff(int s) {
+    int o = 0;
+    for (int i = 0; i < s; ++i) o += prod(i) > 1 ? prod(i) * (RR[i] - 1) : 0;
+    return o;
+  }
+  static constexpr int n_ = N;
+  static constexpr int FIRST = RR[0];
+  static constexpr int LAST = RR[L - 1];
+  static constexpr int TABLE = toff(L);
+  template <int I>
+  using LayOf = typename std::tuple_element<I, std::tuple<Ls...>>::type;
+  template <int CG, int NT>
+  using G1 = Geo<N, FIRST, CG, NT>;
+  template <int CG, int NT>
+  using GL = Geo<N, LAST, CG, NT>;
+  // LDS elements the buffers of a transform of CG interleaved sequences need
+  static constexpr int lds_elems(int cg) {
+    int m = 0;
+    for (int e : {Ls::elems(N * cg)...}) m = e > m ? e : m;
+    return m;
+  }
+  // first-stage butterflies in v (inputs) -> LDS; caller synchronises afterwards
+  template <int DIR, int CG, int NT>
+  __device__ __forceinline__ static void first(float2* buf, float2 (&v)[G1<CG, NT>::MAXB][FIRST]) {
+#pragma unroll
+    for (int b = 0; b < G1<CG, NT>::MAXB; ++b)
+      if (G1<CG, NT>::live(b)) dft<FIRST, DIR>(v[b]);
+    swrite<FIRST, 1, N, CG, NT, LayOf<0>>(buf, v);
+  }
+  template <int S, int DIR, int CG, int NT, bool TAB>
+  __device__ __forceinline__ static void mid_from(float2* buf, const float2* tw, int nh,
+                                                  const float2* tt) {
+    if constexpr (S < L - 1) {
+      cstage<RR[S], prod(S), N, DIR, CG, NT, TAB, LayOf<S - 1>, LayOf<S>>(
+          buf, tw, nh, TAB ? tt + toff(S) : tt);
+      mid_from<S + 1, DIR, CG, NT, TAB>(buf, tw, nh, tt);
+    }
+  }
+  // middle stages (each ends with a barrier)
+  template <int DIR, int CG, int NT, bool TAB = false>
+  __device__ __forceinline__ static void middle(float2* buf, const float2* tw, int nh,
+                                                const float2* tt = nullptr) {
+    mid_from<1, DIR, CG, NT, TAB>(buf, tw, nh, tt);
+  }
+  // last stage -> registers (no barrier)
+  template <int DIR, int CG, int NT, bool TAB = false>
+  __device__ __forceinline__ static void last(const float2* buf, const float2* tw, int nh,
+                                              float2 (&v)[GL<CG, NT>::MAXB][LAST],
+                                              const float2* tt = nullptr) {
+    sread<LAST, prod(L - 1), N, DIR, CG, NT, TAB, LayOf<L - 2>>(
+        buf, tw, nh, TAB ? tt + toff(L - 1) : tt, v);
+  }
+  // entry idx (< TABLE) of the stage tables, from FP64 roots
+  template <int S = 1>
+  __device__ __forceinline__ static void table_entry(float2* out, int idx) {
+    if constexpr (S < L) {
+      constexpr int NS = prod(S), R = RR[S], OFF = toff(S), E = NS * (R - 1);
+      if (idx >= OFF && idx < OFF + E) {
+        const int r = (idx - OFF) / NS + 1, k = (idx - OFF) % NS;
+        const long m = ((long)r * k) % (NS * R);
+        double sn, cs;
+        sincospi(2.0 * (double)m / (double)(NS * R), &sn, &cs);
+        out[idx] = make_float2((float)cs, (float)-sn);
+        return;
+      }
+      table_entry<S + 1>(out, idx);
+    }
+  }
+};
+
+// Runtime<MAXR, LCAP>: any supported length up to LCAP / CG, stage list read from the plan.
+template <int MAXR, int LCAP>
+struct Runtime {
+  static constexpr int n = 0;
+  template <int DIR, int CG, int NT>
+  __device__ __forceinline__ static void exec(float2* buf, const Plan& p, const float2* tw) {
+    run<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
+  }
+};
 
 }  // namespace fft
 }  // namespace cvqpg

@@ -8,7 +8,9 @@ Bars (relative L2, FP32 kernels vs FP64 oracle with an FP64 transfer-function ph
     pipeline;
   * at full 1080p size, the z = 0 plane equals |glue(field)|^2 (a size-independent
     identity of the transform pair).
-FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13.
+FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13 on runtime plans, and the
+compile-time plans of the three standard workloads (transform lengths 1152; 2816 x 4480;
+4992 x 8320) at their real sizes.
 """
 import math
 
@@ -85,6 +87,19 @@ def test_forward_and_adjoint_match_fp64(pkg, C, H, W, pH, pW, dists, grid):
     eg = rel(f_dev.grad, h.grad)
     print(f"  gradient rel L2 {eg:.1e}", flush=True)
     assert eg <= 1e-4
+
+
+FIXED_CASES = [  # (C, H, W, pH, pW, distances): the guard bands of the standard workloads
+    (1, 256, 256, 576, 576, [0.0, 2.5e-3, 5e-3]),      # 1152 rows and columns
+    (3, 1080, 1920, 1408, 2240, [0.0, 6e-3]),           # 2816 columns, 4480 rows
+    (1, 2160, 3840, 2496, 4160, [5e-3]),                # 4992 columns, 8320 rows
+]
+
+
+@pytest.mark.parametrize("C,H,W,pH,pW,dists", FIXED_CASES, ids=["256sq", "1080p", "2160p"])
+def test_fixed_plan_sizes_match_fp64(pkg, C, H, W, pH, pW, dists):
+    """Full-size forward and adjoint on the compile-time transform plans."""
+    test_forward_and_adjoint_match_fp64(pkg, C, H, W, pH, pW, dists, "odak")
 
 
 def test_aperture_and_band_limit_active(pkg):
