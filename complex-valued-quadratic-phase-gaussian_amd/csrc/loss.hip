@@ -19,18 +19,16 @@
 //   A = dS/dmu_x (total), B = dS/dE[x^2], Cc = dS/dE[xy]
 // so dSSIM/dI = G^T A + 2 I G^T B + T G^T Cc.
 //
-// Two kernels: k_ssim_maps (valid-map tiles of 32x32 with an 11-row/col halo in LDS,
-// five separable filters) and k_loss_grad (pixel tiles of 32x32, three transposed
-// filters, the L2 terms, the clamp mask and the target noise).  Block partial sums give
-// the loss value without a host read-back.
+// One streaming kernel (k_loss_stream, described below) computes the SSIM maps, the
+// coefficient maps, the transposed filters and the L2 terms in a single pass over the
+// intensities, target and mask; block partial sums give the loss value without a host
+// read-back (k_loss_total).
 #include "common.h"
 #include "cvqpg.h"
 
 namespace cvqpg {
 
 constexpr int WIN = 11;
-constexpr int TS = 32;                 // output tile
-constexpr int TH = TS + WIN - 1;       // tile + halo (42)
 constexpr float SSIM_C1 = 0.0001f;     // (0.01 * 1)^2
 constexpr float SSIM_C2 = 0.0009f;     // (0.03 * 1)^2
 constexpr float LAMBDA = 0.025f * 0.2f;
@@ -57,8 +55,12 @@ __device__ __forceinline__ void philox_round(uint32_t& c0, uint32_t& c1, uint32_
   c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
 }
 
-__device__ __forceinline__ float normal_at(unsigned long long seed, uint32_t step, uint32_t plane,
-                                           uint32_t idx) {
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+// target noise: one Philox4x32-10 draw gives the four normals of pixels (y, x) with y in
+// 4 g .. 4 g + 3 (two Box-Muller pairs); the counter is (g W + x, plane, step).
+__device__ __forceinline__ void normals4(unsigned long long seed, uint32_t step, uint32_t plane,
+                                         uint32_t idx, float (&z)[4]) {
   uint32_t c0 = idx, c1 = plane, c2 = step, c3 = 0x5EED0001u;
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
 #pragma unroll
@@ -69,211 +71,270 @@ __device__ __forceinline__ float normal_at(unsigned long long seed, uint32_t ste
   }
   const float u1 = ((float)(c0 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float u2 = (float)(c1 >> 8) * (1.0f / 16777216.0f);
-  return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
-}
-
-__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
-
-// ---------------------------------------------------------------------------------
-// SSIM maps: S (summed per block) and the coefficient maps A, B, Cc (pre-multiplied by
-// dL/dS).  grid (ceil(Wv/32), ceil(Hv/32), D*C), 256 threads
-// ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ssim_maps(const float* __restrict__ inten,
-                                                   const float* __restrict__ target,
-                                                   LossParams P, float* __restrict__ mapA,
-                                                   float* __restrict__ mapB,
-                                                   float* __restrict__ mapC,
-                                                   float* __restrict__ ssim_part) {
-  __shared__ float sx[TH][TH + 1];
-  __shared__ float st[TH][TH + 1];
-  __shared__ float hf[5][TH][TS + 1];
-  __shared__ float red[256];
-  const int dc = blockIdx.z;
-  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
-  const size_t plane = (size_t)dc * P.H * P.W;
-  const int t = threadIdx.x;
-  for (int e = t; e < TH * TH; e += 256) {
-    const int yy = e / TH, xx = e % TH;
-    const int gy = y0 + yy, gx = x0 + xx;
-    float a = 0.f, b = 0.f;
-    if (gy < P.H && gx < P.W) {
-      a = clamp01(inten[plane + (size_t)gy * P.W + gx]);
-      b = target[plane + (size_t)gy * P.W + gx];
-    }
-    sx[yy][xx] = a;
-    st[yy][xx] = b;
-  }
-  __syncthreads();
-  // horizontal pass: 42 rows x 32 outputs x 5 products
-  for (int e = t; e < TH * TS; e += 256) {
-    const int yy = e / TS, xx = e % TS;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
-#pragma unroll
-    for (int b = 0; b < WIN; ++b) {
-      const float w = P.win[b];
-      const float a = sx[yy][xx + b], c = st[yy][xx + b];
-      m0 = fmaf(w, a, m0);
-      m1 = fmaf(w, c, m1);
-      m2 = fmaf(w, a * a, m2);
-      m3 = fmaf(w, c * c, m3);
-      m4 = fmaf(w, a * c, m4);
-    }
-    hf[0][yy][xx] = m0;
-    hf[1][yy][xx] = m1;
-    hf[2][yy][xx] = m2;
-    hf[3][yy][xx] = m3;
-    hf[4][yy][xx] = m4;
-  }
-  __syncthreads();
-  float ssum = 0.f;
-  for (int e = t; e < TS * TS; e += 256) {
-    const int yy = e / TS, xx = e % TS;
-    const int qy = y0 + yy, qx = x0 + xx;
-    if (qy >= P.Hv || qx >= P.Wv) continue;
-    float mu1 = 0.f, mu2 = 0.f, e11 = 0.f, e22 = 0.f, e12 = 0.f;
-#pragma unroll
-    for (int a = 0; a < WIN; ++a) {
-      const float w = P.win[a];
-      mu1 = fmaf(w, hf[0][yy + a][xx], mu1);
-      mu2 = fmaf(w, hf[1][yy + a][xx], mu2);
-      e11 = fmaf(w, hf[2][yy + a][xx], e11);
-      e22 = fmaf(w, hf[3][yy + a][xx], e22);
-      e12 = fmaf(w, hf[4][yy + a][xx], e12);
-    }
-    const float s11 = e11 - mu1 * mu1, s22 = e22 - mu2 * mu2, s12 = e12 - mu1 * mu2;
-    const float A1 = 2.f * mu1 * mu2 + SSIM_C1, A2 = 2.f * s12 + SSIM_C2;
-    const float B1 = mu1 * mu1 + mu2 * mu2 + SSIM_C1, B2 = s11 + s22 + SSIM_C2;
-    const float S = (A1 * A2) / (B1 * B2);
-    ssum += S;
-    const float k = P.ssim_coef;
-    // total derivative with respect to mu_x, E[x^2], E[xy]
-    const float dA = S * (2.f * mu2 / A1 - 2.f * mu1 / B1) + (S / B2) * 2.f * mu1 -
-                     (2.f * S / A2) * mu2;
-    const float dB = -S / B2;
-    const float dC = 2.f * S / A2;
-    const size_t q = (size_t)dc * P.Hv * P.Wv + (size_t)qy * P.Wv + qx;
-    mapA[q] = k * dA;
-    mapB[q] = k * dB;
-    mapC[q] = k * dC;
-  }
-  red[t] = ssum;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off) red[t] += red[t + off];
-    __syncthreads();
-  }
-  if (t == 0)
-    ssim_part[((size_t)dc * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+  const float u3 = ((float)(c2 >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float u4 = (float)(c3 >> 8) * (1.0f / 16777216.0f);
+  const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
+  float s1, c1f, s2, c2f;
+  sincospif(2.0f * u2, &s1, &c1f);
+  sincospif(2.0f * u4, &s2, &c2f);
+  z[0] = r1 * c1f; z[1] = r1 * s1; z[2] = r2 * c2f; z[3] = r2 * s2;
 }
 
 // ---------------------------------------------------------------------------------
-// gradient: transposed SSIM filters + L2 terms + clamp mask.  grid (ceil(W/32),
-// ceil(H/32), D*C), 256 threads
+// One streaming pass per (plane, channel, column strip, row segment).
+//
+// Thread i of a workgroup of NTH owns SSIM-map column mc = x0 - 10 + i and, for i >= 10,
+// pixel column mc.  The workgroup walks down the rows of its segment; at each step it
+//   1. stages input row y (clamped intensity and target, NTH + 10 columns) in LDS; the
+//      row was loaded into registers during the previous step,
+//   2. filters it horizontally into the five moments and pushes them into an 11-row
+//      register ring,
+//   3. filters the ring vertically for map row q = y - 10 -> SSIM and the coefficient
+//      maps A, B, Cc of that row (never stored to HBM: staged in LDS for one step),
+//   4. applies the transposed horizontal filter to the map row and pushes the result
+//      into a second 11-row ring (in LDS, private to the thread's column),
+//   5. for pixel row p = q, applies the transposed vertical filter from that ring and
+//      adds the L2 terms, the clamp mask and the target noise: dL/dI.  The pixel's
+//      intensity, target and mask were loaded during the previous step.
+// The rings are indexed by the step modulo 11, which the 11-fold unrolled step loop
+// makes a compile-time choice.  Pixel rows [p0, p0 + RS) need map rows [p0 - 10,
+// p0 + RS) and input rows [p0 - 10, p0 + RS + 10): RS + 20 steps.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_loss_grad(
+#ifndef CVQPG_LOSS_NTH
+#define CVQPG_LOSS_NTH 128
+#endif
+constexpr int NTH = CVQPG_LOSS_NTH;   // threads = map columns per strip
+constexpr int OWN = NTH - (WIN - 1);  // pixel columns per strip
+#ifndef CVQPG_LOSS_RS
+#define CVQPG_LOSS_RS 192
+#endif
+constexpr int RS = CVQPG_LOSS_RS;     // pixel rows per segment
+constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
+
+__global__ __launch_bounds__(NTH) void k_loss_stream(
     const float* __restrict__ inten, const float* __restrict__ target,
     const float* __restrict__ mask, const float* __restrict__ noise,
-    const float* __restrict__ mapA, const float* __restrict__ mapB,
-    const float* __restrict__ mapC, const int* __restrict__ step_counter, LossParams P,
-    float* __restrict__ grad, float* __restrict__ mse_part) {
-  __shared__ float sm[3][TH][TH + 1];
-  __shared__ float hf[3][TH][TS + 1];
-  __shared__ float red[256];
+    const int* __restrict__ step_counter, LossParams P, float* __restrict__ grad,
+    float* __restrict__ ssim_part, float* __restrict__ mse_part) {
+  __shared__ float smo[2][5][NIN];   // staged input row: x, t, x^2, t^2, x t
+  __shared__ float smap[2][3][NTH];
+  __shared__ float gring[WIN][3][NTH];
+  __shared__ float red[2][NTH];
+  const int i = threadIdx.x;
+  const int x0 = blockIdx.x * OWN;
+  const int p0 = blockIdx.y * RS;
   const int dc = blockIdx.z;
-  const int d = dc / P.C;
-  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
-  const int t = threadIdx.x;
-  const size_t mb = (size_t)dc * P.Hv * P.Wv;
-  // coefficient maps over q in [p - 10, p]: tile origin (y0 - 10, x0 - 10)
-  for (int e = t; e < TH * TH; e += 256) {
-    const int yy = e / TH, xx = e % TH;
-    const int qy = y0 - (WIN - 1) + yy, qx = x0 - (WIN - 1) + xx;
-    float a = 0.f, b = 0.f, c = 0.f;
-    if (qy >= 0 && qx >= 0 && qy < P.Hv && qx < P.Wv) {
-      const size_t q = mb + (size_t)qy * P.Wv + qx;
-      a = mapA[q];
-      b = mapB[q];
-      c = mapC[q];
-    }
-    sm[0][yy][xx] = a;
-    sm[1][yy][xx] = b;
-    sm[2][yy][xx] = c;
-  }
-  __syncthreads();
-  // horizontal transposed filter: out(x) = sum_b w_b m(x - b), tile column xx + 10 - b
-  for (int e = t; e < TH * TS; e += 256) {
-    const int yy = e / TS, xx = e % TS;
-    float h0 = 0.f, h1 = 0.f, h2 = 0.f;
-#pragma unroll
-    for (int b = 0; b < WIN; ++b) {
-      const float w = P.win[b];
-      h0 = fmaf(w, sm[0][yy][xx + WIN - 1 - b], h0);
-      h1 = fmaf(w, sm[1][yy][xx + WIN - 1 - b], h1);
-      h2 = fmaf(w, sm[2][yy][xx + WIN - 1 - b], h2);
-    }
-    hf[0][yy][xx] = h0;
-    hf[1][yy][xx] = h1;
-    hf[2][yy][xx] = h2;
-  }
-  __syncthreads();
+  const int d = dc / P.C, c = dc % P.C;
+  const int mc = x0 - (WIN - 1) + i;              // map column (and pixel column, i >= 10)
+  const bool own_col = i >= WIN - 1;
+  const bool map_col = mc >= 0 && mc < P.Wv;
+  const bool pix_col = own_col && mc < P.W;
+  // plane offsets in 64 bits, offsets inside a plane (< 2^31 elements) in 32 bits
+  const float* inp = inten + (size_t)dc * P.H * P.W;
+  const float* tgp = target + (size_t)dc * P.H * P.W;
+  const float* mkp = mask + (size_t)dc * P.H * P.W;
+  const float* nzp = noise ? noise + (size_t)dc * P.H * P.W : nullptr;
+  float* grp = grad + (size_t)dc * P.H * P.W;
+  const int Wd = P.W;
   const uint32_t step = step_counter ? (uint32_t)step_counter[0] : 0u;
   const float ns = P.noise_scale[d];
-  const size_t plane = (size_t)dc * P.H * P.W;
-  const size_t cplane = (size_t)(dc % P.C) * P.H * P.W;
-  float msum = 0.f;
-  for (int e = t; e < TS * TS; e += 256) {
-    const int yy = e / TS, xx = e % TS;
-    const int py = y0 + yy, px = x0 + xx;
-    if (py >= P.H || px >= P.W) continue;
-    float gA = 0.f, gB = 0.f, gC = 0.f;
+  const int p_end = min(p0 + RS, P.H);
+  const int ys = p0 - (WIN - 1);                  // first input row
+  const int nsteps = RS + 2 * (WIN - 1);
+  const uint32_t hq = (uint32_t)((P.H + 3) / 4);
+  // the input columns this thread stages: x0 - 10 + i and, for i < 10, x0 - 10 + i + NTH
+  const int gx0 = mc, gx1 = mc + NTH;
+  const bool two = i < WIN - 1;
+  float pa0 = 0.f, pb0 = 0.f, pa1 = 0.f, pb1 = 0.f;   // next input row (raw inten, target)
+  float qI = 0.f, qT = 0.f, qM = 0.f, qN = 0.f;       // next pixel row values
+  auto load_row = [&](int y) {
+    pa0 = pb0 = pa1 = pb1 = 0.f;
+    if (y >= 0 && y < P.H) {
+      const int r = y * Wd;
+      if (gx0 >= 0 && gx0 < Wd) { pa0 = inp[r + gx0]; pb0 = tgp[r + gx0]; }
+      if (two && gx1 < Wd) { pa1 = inp[r + gx1]; pb1 = tgp[r + gx1]; }
+    }
+  };
+  auto load_pix = [&](int pr) {
+    if (pix_col && pr >= p0 && pr < p_end) {
+      const int r = pr * Wd + mc;
+      qI = inp[r]; qT = tgp[r]; qM = mkp[r];
+      if (nzp) qN = nzp[r];
+    }
+  };
+  float hm[WIN][5];   // moment ring: row-filtered x, t, x^2, t^2, xt of input rows
 #pragma unroll
-    for (int a = 0; a < WIN; ++a) {
-      const float w = P.win[a];
-      gA = fmaf(w, hf[0][yy + WIN - 1 - a][xx], gA);
-      gB = fmaf(w, hf[1][yy + WIN - 1 - a][xx], gB);
-      gC = fmaf(w, hf[2][yy + WIN - 1 - a][xx], gC);
+  for (int k = 0; k < WIN; ++k)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) hm[k][m] = 0.f;
+#pragma unroll
+  for (int k = 0; k < WIN; ++k)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) gring[k][m][i] = 0.f;
+  float z4[4] = {0.f, 0.f, 0.f, 0.f};
+  float ssum = 0.f, msum = 0.f;
+  load_row(ys);
+
+  for (int base = 0; base < nsteps; base += WIN) {
+#pragma unroll
+    for (int k = 0; k < WIN; ++k) {
+      const int sidx = base + k;
+      if (sidx >= nsteps) break;
+      const int y = ys + sidx;
+      const int buf = sidx & 1;
+      // 1. stage input row y (loaded last step), then start loading row y + 1 and the
+      //    pixel values of pixel row y - 9 (used next step)
+      {
+        const float a = clamp01(pa0), b = pb0;
+        smo[buf][0][i] = a; smo[buf][1][i] = b;
+        smo[buf][2][i] = a * a; smo[buf][3][i] = b * b; smo[buf][4][i] = a * b;
+        if (two) {
+          const float a1 = clamp01(pa1), b1 = pb1;
+          smo[buf][0][i + NTH] = a1; smo[buf][1][i + NTH] = b1;
+          smo[buf][2][i + NTH] = a1 * a1; smo[buf][3][i + NTH] = b1 * b1;
+          smo[buf][4][i + NTH] = a1 * b1;
+        }
+      }
+      const float cI = qI, cT = qT, cM = qM, cN = qN;
+      load_row(y + 1);
+      load_pix(y - (WIN - 1) + 1);
+      __syncthreads();
+      // 2. horizontal moments of map column mc (input columns mc .. mc + 10)
+      {
+        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+#pragma unroll
+        for (int b = 0; b < WIN; ++b) {
+          const float w = P.win[b];
+          m0 = fmaf(w, smo[buf][0][i + b], m0);
+          m1 = fmaf(w, smo[buf][1][i + b], m1);
+          m2 = fmaf(w, smo[buf][2][i + b], m2);
+          m3 = fmaf(w, smo[buf][3][i + b], m3);
+          m4 = fmaf(w, smo[buf][4][i + b], m4);
+        }
+        hm[k][0] = m0; hm[k][1] = m1; hm[k][2] = m2; hm[k][3] = m3; hm[k][4] = m4;
+      }
+      if (sidx < WIN - 1) continue;          // ring not yet full: no map row
+      // 3. map row q = y - 10 from input rows q .. q + 10 (ring slots k + 1 + a mod 11)
+      const int q = y - (WIN - 1);
+      float cA = 0.f, cB = 0.f, cC = 0.f;
+      if (q >= 0 && q < P.Hv && map_col) {
+        float mu1 = 0.f, mu2 = 0.f, e11 = 0.f, e22 = 0.f, e12 = 0.f;
+#pragma unroll
+        for (int a = 0; a < WIN; ++a) {
+          const int sl = (k + 1 + a) % WIN;
+          const float w = P.win[a];
+          mu1 = fmaf(w, hm[sl][0], mu1);
+          mu2 = fmaf(w, hm[sl][1], mu2);
+          e11 = fmaf(w, hm[sl][2], e11);
+          e22 = fmaf(w, hm[sl][3], e22);
+          e12 = fmaf(w, hm[sl][4], e12);
+        }
+        const float s11 = e11 - mu1 * mu1, s22 = e22 - mu2 * mu2, s12 = e12 - mu1 * mu2;
+        const float A1 = 2.f * mu1 * mu2 + SSIM_C1, A2 = 2.f * s12 + SSIM_C2;
+        const float B1 = mu1 * mu1 + mu2 * mu2 + SSIM_C1, B2 = s11 + s22 + SSIM_C2;
+        // one hardware reciprocal (1 ulp) of B1 B2 > 0; S / A1 and S / A2 are written
+        // as A2 / (B1 B2) and A1 / (B1 B2), so A1 or A2 = 0 needs no special case
+        const float rB = __builtin_amdgcn_rcpf(B1 * B2);
+        const float S = (A1 * A2) * rB;
+        if (own_col && q >= p0) ssum += S;
+        const float kc = P.ssim_coef;
+        // total derivative with respect to mu_x, E[x^2], E[xy]:
+        //   dS/dmu_x = 2 mu_y (A2 - A1) / (B1 B2) + 2 mu_x S (B1 - B2) / (B1 B2)
+        //   dS/dE[x^2] = -S / B2,  dS/dE[xy] = 2 A1 / (B1 B2)
+        cA = 2.f * kc * rB * (mu2 * (A2 - A1) + mu1 * S * (B1 - B2));
+        cB = -kc * S * B1 * rB;
+        cC = 2.f * kc * A1 * rB;
+      }
+      smap[buf][0][i] = cA;
+      smap[buf][1][i] = cB;
+      smap[buf][2][i] = cC;
+      __syncthreads();
+      // 4. transposed horizontal filter: map columns mc - 10 .. mc (threads i - 10 .. i)
+      if (!own_col) continue;
+      {
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+#pragma unroll
+        for (int b = 0; b < WIN; ++b) {
+          const float w = P.win[b];
+          g0 = fmaf(w, smap[buf][0][i - b], g0);
+          g1 = fmaf(w, smap[buf][1][i - b], g1);
+          g2 = fmaf(w, smap[buf][2][i - b], g2);
+        }
+        gring[k][0][i] = g0; gring[k][1][i] = g1; gring[k][2][i] = g2;
+      }
+      // 5. pixel row p = q: transposed vertical filter over map rows p - a (slots k - a)
+      const int pr = q;
+      if (pr < p0 || pr >= p_end) continue;
+      if (ns != 0.f && !noise && ((pr - p0) & 3) == 0 && pix_col)
+        normals4(P.seed, step, (uint32_t)d,
+                 ((uint32_t)c * hq + (uint32_t)(pr >> 2)) * (uint32_t)Wd + (uint32_t)mc, z4);
+      if (!pix_col) continue;
+      float gA = 0.f, gB = 0.f, gC = 0.f;
+#pragma unroll
+      for (int a = 0; a < WIN; ++a) {
+        const int sl = (k - a + WIN) % WIN;
+        const float w = P.win[a];
+        gA = fmaf(w, gring[sl][0][i], gA);
+        gB = fmaf(w, gring[sl][1][i], gB);
+        gC = fmaf(w, gring[sl][2][i], gC);
+      }
+      const float I = clamp01(cI);
+      float nz = 0.f;
+      if (ns != 0.f) {
+        const int l = (pr - p0) & 3;
+        nz = noise ? cN : (l == 0 ? z4[0] : l == 1 ? z4[1] : l == 2 ? z4[2] : z4[3]);
+        nz *= ns;
+      }
+      const float Tn = cT + nz;
+      const float e1 = I - cT, e2 = I - Tn;
+      const float wgt = cM * cM + Tn * Tn;
+      msum += e1 * e1 + wgt * e2 * e2;
+      const float gM = 2.f * P.inv_n * (e1 + wgt * e2);
+      const float gS = gA + 2.f * I * gB + cT * gC;
+      grp[pr * Wd + mc] = (cI >= 0.f && cI <= 1.f) ? gM + gS : 0.f;
     }
-    const size_t pi = plane + (size_t)py * P.W + px;
-    const float raw = inten[pi];
-    const float I = clamp01(raw);
-    const float T = target[pi];
-    const float M = mask[pi];
-    float nz = 0.f;
-    if (ns != 0.f) {
-      nz = noise ? noise[pi] : normal_at(P.seed, step, (uint32_t)d,
-                                         (uint32_t)(cplane + (size_t)py * P.W + px));
-      nz *= ns;
-    }
-    const float Tn = T + nz;
-    const float e1 = I - T, e2 = I - Tn;
-    msum += e1 * e1 + (M * M + Tn * Tn) * e2 * e2;
-    const float gM = 2.f * P.inv_n * (e1 + (M * M + Tn * Tn) * e2);
-    const float gS = gA + 2.f * I * gB + T * gC;
-    grad[pi] = (raw >= 0.f && raw <= 1.f) ? gM + gS : 0.f;
   }
-  red[t] = msum;
+  red[0][i] = ssum;
+  red[1][i] = msum;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (t < off) red[t] += red[t + off];
+  for (int off = NTH / 2; off > 0; off >>= 1) {
+    if (i < off) {
+      red[0][i] += red[0][i + off];
+      red[1][i] += red[1][i + off];
+    }
     __syncthreads();
   }
-  if (t == 0)
-    mse_part[((size_t)dc * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+  if (i == 0) {
+    const size_t b = ((size_t)dc * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    ssim_part[b] = red[0][0];
+    mse_part[b] = red[1][0];
+  }
 }
 
-// loss = sum_d [ mse_d / n + LAMBDA (1 - SSIM_d) ]
-__global__ __launch_bounds__(256) void k_loss_total(const float* __restrict__ ssim_part,
-                                                    int n_ssim, const float* __restrict__ mse_part,
-                                                    int n_mse, LossParams P,
-                                                    float* __restrict__ loss) {
-  __shared__ double red[256];
+// loss = sum_d [ mse_d / n + LAMBDA (1 - SSIM_d) ]; one workgroup, fixed summation order
+__global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ ssim_part,
+                                                     int n_ssim, const float* __restrict__ mse_part,
+                                                     int n_mse, LossParams P,
+                                                     float* __restrict__ loss) {
+  __shared__ double red[1024];
   const int t = threadIdx.x;
-  double s = 0.0, m = 0.0;
-  for (int i = t; i < n_ssim; i += 256) s += ssim_part[i];
-  for (int i = t; i < n_mse; i += 256) m += mse_part[i];
-  red[t] = (double)P.inv_n * m - (double)LAMBDA / ((double)P.C * P.Hv * P.Wv) * s;
+  double s[4] = {0.0, 0.0, 0.0, 0.0}, m[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = t;
+  for (; i + 3 * 1024 < n_ssim; i += 4 * 1024) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s[u] += ssim_part[i + u * 1024];
+  }
+  for (; i < n_ssim; i += 1024) s[0] += ssim_part[i];
+  i = t;
+  for (; i + 3 * 1024 < n_mse; i += 4 * 1024) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) m[u] += mse_part[i + u * 1024];
+  }
+  for (; i < n_mse; i += 1024) m[0] += mse_part[i];
+  red[t] = (double)P.inv_n * ((m[0] + m[1]) + (m[2] + m[3])) -
+           (double)LAMBDA / ((double)P.C * P.Hv * P.Wv) * ((s[0] + s[1]) + (s[2] + s[3]));
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = 512; off > 0; off >>= 1) {
     if (t < off) red[t] += red[t + off];
     __syncthreads();
   }
@@ -281,23 +342,22 @@ __global__ __launch_bounds__(256) void k_loss_total(const float* __restrict__ ss
 }
 
 struct LossWS {
-  float* mapA;
-  float* mapB;
-  float* mapC;
   float* ssim_part;
   float* mse_part;
   size_t bytes;
 };
 
+static dim3 loss_grid(int C, int D, int H, int W) {
+  return dim3(ceil_div(W, OWN), ceil_div(H, RS), D * C);
+}
+
 static LossWS carve_loss(void* base, int C, int D, int H, int W) {
   Carver cv(base);
   LossWS w;
-  const size_t nv = (size_t)D * C * (H - WIN + 1) * (W - WIN + 1);
-  w.mapA = cv.take<float>(nv);
-  w.mapB = cv.take<float>(nv);
-  w.mapC = cv.take<float>(nv);
-  w.ssim_part = cv.take<float>((size_t)D * C * ceil_div(H, TS) * ceil_div(W, TS));
-  w.mse_part = cv.take<float>((size_t)D * C * ceil_div(H, TS) * ceil_div(W, TS));
+  const dim3 g = loss_grid(C, D, H, W);
+  const size_t nb = (size_t)g.x * g.y * g.z;
+  w.ssim_part = cv.take<float>(nb);
+  w.mse_part = cv.take<float>(nb);
   w.bytes = cv.bytes();
   return w;
 }
@@ -347,14 +407,12 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
   P.ssim_coef = (float)(-(double)LAMBDA / ((double)channels * P.Hv * P.Wv));
   for (int d = 0; d < 16; ++d) P.noise_scale[d] = (noise_scale && d < planes) ? noise_scale[d] : 0.f;
   P.seed = seed;
-  const dim3 gv(ceil_div(P.Wv, TS), ceil_div(P.Hv, TS), planes * channels);
-  const dim3 gp(ceil_div(width, TS), ceil_div(height, TS), planes * channels);
-  hipLaunchKernelGGL(k_ssim_maps, gv, dim3(256), 0, s, intensity, target, P, w.mapA, w.mapB,
-                     w.mapC, w.ssim_part);
-  hipLaunchKernelGGL(k_loss_grad, gp, dim3(256), 0, s, intensity, target, mask, noise, w.mapA,
-                     w.mapB, w.mapC, step_counter, P, grad_intensity, w.mse_part);
-  hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(256), 0, s, w.ssim_part,
-                     (int)(gv.x * gv.y * gv.z), w.mse_part, (int)(gp.x * gp.y * gp.z), P, loss);
+  const dim3 g = loss_grid(channels, planes, height, width);
+  const int nb = (int)(g.x * g.y * g.z);
+  hipLaunchKernelGGL(k_loss_stream, g, dim3(NTH), 0, s, intensity, target, mask, noise,
+                     step_counter, P, grad_intensity, w.ssim_part, w.mse_part);
+  hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(1024), 0, s, w.ssim_part, nb, w.mse_part, nb,
+                     P, loss);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }

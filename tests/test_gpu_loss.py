@@ -46,7 +46,8 @@ def oracle_loss(inten, target, mask, noise, scale):
     return float(total), I.grad
 
 
-@pytest.mark.parametrize("C,D,H,W", [(3, 4, 40, 56), (1, 3, 64, 64), (3, 2, 33, 47)])
+@pytest.mark.parametrize("C,D,H,W", [(3, 4, 40, 56), (1, 3, 64, 64), (3, 2, 33, 47), (1, 2, 75, 139),
+                                     (3, 1, 1080, 1920)])
 def test_loss_and_gradient_match_fp64(pkg, C, D, H, W):
     rng = np.random.default_rng(C * 100 + H)
     inten = rng.uniform(-0.1, 1.2, (D, C, H, W)).astype(np.float32)
