@@ -35,8 +35,19 @@ constexpr int MAXD = 16;
 #ifndef CVQPG_COL_NT
 #define CVQPG_COL_NT 512
 #endif
-#ifndef CVQPG_WPE
-#define CVQPG_WPE 1
+// waves per SIMD the 2816-point column kernels are compiled for: the forward runs
+// faster at 4 (128 VGPRs, a small spill) than at 2, the adjoint spills too much at 4
+#ifndef CVQPG_WPE_F
+#define CVQPG_WPE_F 4
+#endif
+#ifndef CVQPG_WPE_A
+#define CVQPG_WPE_A 1
+#endif
+#ifndef CVQPG_PG_RECOMPUTE
+#define CVQPG_PG_RECOMPUTE 0
+#endif
+#ifndef CVQPG_PG_RECOMPUTE_A
+#define CVQPG_PG_RECOMPUTE_A 0
 #endif
 constexpr int ROW_NT = CVQPG_ROW_NT;
 constexpr int ROW_LCAP = 8704;   // longest row transform (complex elements)
@@ -438,11 +449,11 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
 // spectrum in the last-stage registers, which are the first-stage inputs of the inverse
 // transform (the inverse runs the radix list reversed).
 
-// RowPlan<NT, Fwd, Inv>: row transform of length Fwd::N on NT threads; ColPlan<CG, NT, WPE,
-// Fwd, Inv>: column transform of CG columns on NT threads compiled for at least WPE waves
-// per SIMD.  Inv runs Fwd's radices reversed (its first stage consumes the forward's
-// last-stage registers); the buffer layouts are the ones a 64-bank conflict model picks
-// for each stage boundary.
+// RowPlan<NT, Fwd, Inv>: row transform of length Fwd::N on NT threads; ColPlan<CG, NT,
+// WPE_F, WPE_A, Fwd, Inv>: column transform of CG columns on NT threads, the forward and
+// adjoint kernels compiled for at least WPE_F / WPE_A waves per SIMD.  Inv runs Fwd's
+// radices reversed (its first stage consumes the forward's last-stage registers); the
+// buffer layouts are the ones a 64-bank conflict model picks for each stage boundary.
 template <class F, class I>
 struct PlanPair {
   using Fwd = F;
@@ -455,9 +466,9 @@ struct RowPlan : PlanPair<F, I> {
   static constexpr int nt = NT;
   static constexpr int lds = (F::lds_elems(1) > I::lds_elems(1) ? F::lds_elems(1) : I::lds_elems(1));
 };
-template <int CG, int NT, int WPE, class F, class I>
+template <int CG, int NT, int WPE_F, int WPE_A, class F, class I>
 struct ColPlan : PlanPair<F, I> {
-  static constexpr int cg = CG, nt = NT, wpe = WPE;
+  static constexpr int cg = CG, nt = NT, wpe_f = WPE_F, wpe_a = WPE_A;
   static constexpr int lds = (F::lds_elems(CG) > I::lds_elems(CG) ? F::lds_elems(CG) : I::lds_elems(CG));
 };
 using fft::Pipe;
@@ -693,7 +704,7 @@ __global__ void k_pipe_tables(float2* __restrict__ out) {
 // workgroup; only the rows of the inverse that can fall in the crop ([N/4, 3N/4)) are
 // formed.  grid (8 * per, C)
 template <class P>
-__global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __restrict__ X, Optics o,
+__global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __restrict__ X, Optics o,
                                                               fft::Plan plan,
                                                               const float2* __restrict__ tabs,
                                                               int per, float2* __restrict__ Y) {
@@ -736,8 +747,9 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __re
   __syncthreads();
   Fw::template middle<1, CG, NT>(buf, stw, plan.nh);
   float2 S[FL::MAXB][Fw::LAST];
-  float Pg[FL::MAXB][Fw::LAST];
   Fw::template last<1, CG, NT>(buf, stw, plan.nh, S);
+#if !CVQPG_PG_RECOMPUTE
+  float Pg[FL::MAXB][Fw::LAST];
 #pragma unroll
   for (int b = 0; b < FL::MAXB; ++b) {
     if (!FL::live(b)) continue;
@@ -746,6 +758,7 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __re
 #pragma unroll
     for (int k = 0; k < Fw::LAST; ++k) Pg[b][k] = tf_point(o, c, j + k * FL::CNT, col0 + col);
   }
+#endif
   for (int d = 0; d < o.D; ++d) {
     __syncthreads();   // the previous reads of buf are complete
     {
@@ -756,8 +769,14 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __re
         int j, col;
         FL::item(b, j, col);
 #pragma unroll
-        for (int k = 0; k < Fw::LAST; ++k)
-          v[b][k] = fft::cmul(S[b][k], tf_value(o, Pg[b][k], c, d, j + k * FL::CNT, col0 + col));
+        for (int k = 0; k < Fw::LAST; ++k) {
+#if CVQPG_PG_RECOMPUTE
+          const float pg = tf_point(o, c, j + k * FL::CNT, col0 + col);
+#else
+          const float pg = Pg[b][k];
+#endif
+          v[b][k] = fft::cmul(S[b][k], tf_value(o, pg, c, d, j + k * FL::CNT, col0 + col));
+        }
       }
       Iv::template first<-1, CG, NT>(buf, v);
     }
@@ -784,7 +803,7 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_f2_cols_fx(const float2* __re
 // A2 (see k_a2_cols) with a compile-time plan.  The per-plane forward transforms read
 // their twiddles from the forward pipe's stage tables staged in LDS.  grid (8 * per, C)
 template <class P>
-__global__ __launch_bounds__(P::nt, P::wpe) void k_a2_cols_fx(const float2* __restrict__ Z, Optics o,
+__global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __restrict__ Z, Optics o,
                                                               fft::Plan plan,
                                                               const float2* __restrict__ tabs,
                                                               int per, float2* __restrict__ G) {
@@ -805,7 +824,9 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_a2_cols_fx(const float2* __re
   float2* tt = buf + col_lds_head<P>();
   for (int i = threadIdx.x; i < Fw::TABLE; i += NT) tt[i] = tabs[i];
   float2 A[FL::MAXB][Fw::LAST];
+#if !CVQPG_PG_RECOMPUTE_A
   float Pg[FL::MAXB][Fw::LAST];
+#endif
 #pragma unroll
   for (int b = 0; b < FL::MAXB; ++b) {
     int j, col;
@@ -813,7 +834,9 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_a2_cols_fx(const float2* __re
 #pragma unroll
     for (int k = 0; k < Fw::LAST; ++k) {
       A[b][k] = make_float2(0.f, 0.f);
+#if !CVQPG_PG_RECOMPUTE_A
       Pg[b][k] = FL::live(b) ? tf_point(o, c, j + k * FL::CNT, col0 + col) : -1.0f;
+#endif
     }
   }
   for (int d = 0; d < o.D; ++d) {
@@ -850,7 +873,12 @@ __global__ __launch_bounds__(P::nt, P::wpe) void k_a2_cols_fx(const float2* __re
       FL::item(b, j, col);
 #pragma unroll
       for (int k = 0; k < Fw::LAST; ++k) {
-        const float2 h = tf_value(o, Pg[b][k], c, d, j + k * FL::CNT, col0 + col);
+#if CVQPG_PG_RECOMPUTE_A
+        const float pg = tf_point(o, c, j + k * FL::CNT, col0 + col);
+#else
+        const float pg = Pg[b][k];
+#endif
+        const float2 h = tf_value(o, pg, c, d, j + k * FL::CNT, col0 + col);
         A[b][k].x += v[b][k].x * h.x + v[b][k].y * h.y;   // v * conj(h)
         A[b][k].y += v[b][k].y * h.x - v[b][k].x * h.y;
       }
@@ -921,11 +949,11 @@ using RowP8320 = RowPlan<640, Pipe<8320, Rad<16, 8, 5, 13>, Lays<Ly<32, 1>, Ly<3
 using RowP1152 = RowPlan<128, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 1>, Ly<32, 4>>>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<64, 1>>>>;
 // compile-time column plans (2816: 1080p, 1152: 256^2, 4992: 2160p)
-using ColP2816 = ColPlan<2, 512, CVQPG_WPE, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 2>, Ly0>>,
+using ColP2816 = ColPlan<2, 512, CVQPG_WPE_F, CVQPG_WPE_A, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 2>, Ly0>>,
                          Pipe<2816, Rad<11, 16, 16>, Lays<Ly0, Ly<32, 2>>>>;
-using ColP1152 = ColPlan<4, 512, 2, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 2>, Ly0>>,
+using ColP1152 = ColPlan<4, 512, 2, 2, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 2>, Ly0>>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<128, 2>>>>;
-using ColP4992 = ColPlan<1, 384, 2, Pipe<4992, Rad<16, 8, 3, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
+using ColP4992 = ColPlan<1, 384, 2, 2, Pipe<4992, Rad<16, 8, 3, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
                          Pipe<4992, Rad<13, 3, 8, 16>, Lays<Ly0, Ly0, Ly0>>>;
 
 template <class P>

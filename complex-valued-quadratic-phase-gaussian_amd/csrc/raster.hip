@@ -434,6 +434,9 @@ __global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ran
       const float ex = fmaf(p0.w, dy, p0.z * dx);
       const float ey = fmaf(p1.x, dy, p0.w * dx);
       const float q = fmaf(dy, ey, dx * ex);
+      // pairs beyond the power clamp (power < -50) add o e^-50 ~ 2e-22 per channel;
+      // a wave whose 64 pixels are all there skips the Gaussian
+      if (__ballot(q >= QMIN) == 0ull) continue;
       const float G = fast_exp2(fmaxf(q, QMIN));
       {
         const float rev = fract_f(fmaf(p1.z, q, p1.y));
