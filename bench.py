@@ -34,7 +34,31 @@ def parse():
                          "step (HIP raster, PyTorch elsewhere)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB after the timed steps (a "
+                         "plain copy kernel of known size for scaling HBM byte counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_bwd_traffic_rgb1080p_150k.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_render_bwd launch on the default workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values scaled by the same
+    counters' reading on a plain copy kernel of known size); None if absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def raster_flops_per_pair(C):
@@ -163,10 +187,20 @@ def main():
     ms = 1e3 * dt / args.steps
     pairs = P * 256
     achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
+    traffic = measured_traffic() if args.workload == WORKLOAD else None
     roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)", "bound": "mfma",
             "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": k_ms,
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,
             "pairs": pairs, "flop_per_pair": f_bwd}
+    if traffic is not None:
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if args.calib_copy > 0:
+        n = args.calib_copy * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {args.calib_copy} MiB done", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)

@@ -14,14 +14,16 @@
 //                only the pH non-zero rows of the 2x grid are transformed (X [pH, nv]).
 //   F2  columns: load the pH non-zero entries of CG adjacent columns, FFT along the
 //                column, keep the spectrum in registers, and for every plane d multiply
-//                by H_d (FP64 phase evaluated on the fly, no cached transfer function),
-//                inverse FFT along the column and store only the rows that survive the
-//                final crop (Y_d [oh, nv]).
+//                by H_d (phase evaluated on the fly from a host FP64 carrier and an FP32
+//                residual, no cached transfer function), inverse FFT along the column and
+//                store only the rows that survive the final crop (Y_d [oh, nv]).
 //   F3  rows   : inverse FFT along the row, store only the cropped columns: the complex
 //                field at the plane (kept for the adjoint) and its intensity.
 // The adjoint runs the transposed passes in reverse (A3 rows, A2 columns with the
 // conjugate transfer functions summed over planes, A1 rows + the glue adjoint), so the
-// full 2x spectrum never goes to HBM in either direction.
+// full 2x spectrum never goes to HBM in either direction.  The transform lengths of the
+// standard workloads run on compile-time pipelines (k_*_fx, see fft::Pipe); any other
+// size runs on the runtime-plan kernels.
 #include "common.h"
 #include "cvqpg.h"
 #include "fft.h"

@@ -514,45 +514,70 @@ __global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ran
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) kA[c] = kc[c] * col[c];
+    // Per pixel the quadratic form is q = dx (a00 dx + 2 a01 dy) + a11 dy^2 (two FMAs
+    // from per-row constants), and the pixel sums that need dx or dy are kept as the
+    // row moments S0 = sum h, S1 = sum h dx, S2 = sum h dx^2 of h = dL/dq / o, folded
+    // into the mean and inverse-covariance gradients once per row.  The opacity factor
+    // of alpha = o G is applied once at the end.
     float dcol[C], dph[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) dcol[c] = dph[c] = 0.f;
     float dop = 0.f, K = 0.f, di00 = 0.f, di01 = 0.f, di11 = 0.f, dmx = 0.f, dmy = 0.f;
+    const float dx0 = mx - (float)x0;
     for (int yy = 0; yy < ny; ++yy) {
       const float dy = my - (float)(y0 + yy);
-      for (int xx = 0; xx < nx; ++xx) {
-        const float dx = mx - (float)(x0 + xx);
-        const int p = yy * TILE + xx;
-        const float ex = fmaf(a01, dy, a00 * dx);
-        const float ey = fmaf(a11, dy, a01 * dx);
-        const float q = fmaf(dy, ey, dx * ex);
+      const float bq = 2.0f * a01 * dy, cq = a11 * dy * dy;
+      float S0 = 0.f, S1 = 0.f, S2 = 0.f;
+      const float2* gr0 = &sgr[0][yy * TILE];
+      auto pixel = [&](int xx) {
+        const float dx = dx0 - (float)xx;
+        const float q = fmaf(dx, fmaf(a00, dx, bq), cq);
         const float G = fast_exp2(fmaxf(q, QMIN));
-        const float alpha = o * G;
         float Sa = 0.f, Tb = 0.f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float2 gg = sgr[c][p];
+          const float2 gg = gr0[c * 256 + xx];
           const float rev = fract_f(fmaf(w[c], q, u[c]));
           const float sn = sin_rev(rev), cs = cos_rev(rev);
           const float Pc = fmaf(gg.x, cs, gg.y * sn);
           const float Qc = fmaf(gg.y, cs, -gg.x * sn);
-          dcol[c] = fmaf(alpha, Pc, dcol[c]);
-          dph[c] = fmaf(alpha, Qc, dph[c]);
+          dcol[c] = fmaf(G, Pc, dcol[c]);
+          dph[c] = fmaf(G, Qc, dph[c]);
           Sa = fmaf(col[c], Pc, Sa);
           Tb = fmaf(kA[c], Qc, Tb);
         }
         dop = fmaf(Sa, G, dop);
         const float sa = (q >= QMIN) ? Sa : 0.f;
-        const float dLdm = alpha * fmaf(Tb, beta, -0.5f * sa);
-        K = fmaf(alpha * Tb, q, K);
-        const float tdx = dLdm * dx;
-        di00 = fmaf(tdx, dx, di00);
-        di01 = fmaf(tdx, dy, di01);
-        di11 = fmaf(dLdm * dy, dy, di11);
-        dmx = fmaf(dLdm, ex, dmx);
-        dmy = fmaf(dLdm, ey, dmy);
+        const float h = G * fmaf(Tb, beta, -0.5f * sa);
+        K = fmaf(G * Tb, q, K);
+        S0 += h;
+        const float hd = h * dx;
+        S1 += hd;
+        S2 = fmaf(hd, dx, S2);
+      };
+      if (nx == TILE) {
+#pragma unroll
+        for (int xx = 0; xx < TILE; ++xx) pixel(xx);
+      } else {
+        for (int xx = 0; xx < nx; ++xx) pixel(xx);
       }
+      di00 += S2;
+      di01 = fmaf(dy, S1, di01);
+      di11 = fmaf(dy * dy, S0, di11);
+      dmx += fmaf(a00, S1, a01 * dy * S0);
+      dmy += fmaf(a01, S1, a11 * dy * S0);
     }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      dcol[c] *= o;
+      dph[c] *= o;
+    }
+    K *= o;
+    di00 *= o;
+    di01 *= o;
+    di11 *= o;
+    dmx *= o;
+    dmy *= o;
     if (valid) {
       const int4 rc = rect[g];
       const int e = rc.x + (ty - rc.z) * rc.w + (tx - rc.y);
