@@ -37,13 +37,12 @@ constexpr int MAXD = 16;
 #ifndef CVQPG_COL_NT
 #define CVQPG_COL_NT 512
 #endif
-// waves per SIMD the 2816-point column kernels are compiled for: the forward runs
-// faster at 4 (128 VGPRs, a small spill) than at 2, the adjoint spills too much at 4
+// waves per SIMD the 2816-point column kernels are compiled for (measured best)
 #ifndef CVQPG_WPE_F
 #define CVQPG_WPE_F 4
 #endif
 #ifndef CVQPG_WPE_A
-#define CVQPG_WPE_A 1
+#define CVQPG_WPE_A 3
 #endif
 #ifndef CVQPG_PG_RECOMPUTE
 #define CVQPG_PG_RECOMPUTE 0
@@ -153,10 +152,13 @@ __device__ __forceinline__ float tf_point(const Optics& o, int c, int kr, int kc
   return g;
 }
 
-// H_d / carrier at a point: exp(-i 2 pi (z/lambda) g) inside the band, 0 outside
+// H_d / carrier at a point: exp(-i 2 pi (z/lambda) g) inside the band, 0 outside.
+// BAND = false compiles the point without the band-limit test (no band limit and no
+// evanescence can occur on the grid: Optics::band_active == 0).
+template <bool BAND = true>
 __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int d, int kr, int kc) {
   if (g < 0.0f) return make_float2(0.f, 0.f);
-  if (o.band_active) {
+  if (BAND && o.band_active) {
     int ir, ic;
     float fy, fx;
     tf_freqs(o, kr, kc, ir, ic, fy, fx);
@@ -705,7 +707,7 @@ __global__ void k_pipe_tables(float2* __restrict__ out) {
 // their twiddles from the inverse pipe's stage tables, staged in LDS once per
 // workgroup; only the rows of the inverse that can fall in the crop ([N/4, 3N/4)) are
 // formed.  grid (8 * per, C)
-template <class P>
+template <class P, bool BAND>
 __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __restrict__ X, Optics o,
                                                               fft::Plan plan,
                                                               const float2* __restrict__ tabs,
@@ -726,6 +728,8 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
   const float2* stw = stage_twiddles(plan, buf + P::lds);
   float2* tt = buf + col_lds_head<P>();
   for (int i = threadIdx.x; i < Iv::TABLE; i += NT) tt[i] = tabs[Fw::TABLE + i];
+  // plane bases are uniform; offsets inside a plane (< 2^32 elements) are 32-bit
+  const float2* Xc = X + (size_t)c * o.pH * o.nv + col0;
   {
     float2 v[F1::MAXB][Fw::FIRST];
 #pragma unroll
@@ -733,13 +737,12 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
       if (!F1::live(b)) continue;
       int j, col;
       F1::item(b, j, col);
-      const float2* xb = X + ((ptrdiff_t)c * o.pH + j - o.zr0) * o.nv + col0 + col;
 #pragma unroll
       for (int k = 0; k < Fw::FIRST; ++k) {
         float2 u = make_float2(0.f, 0.f);
         if (in_mid_half<Fw>(k)) {
           const int rr = j + k * F1::CNT - o.zr0;
-          if ((unsigned)rr < (unsigned)o.pH) u = xb[(size_t)(k * F1::CNT) * o.nv];
+          if ((unsigned)rr < (unsigned)o.pH) u = Xc[(unsigned)(rr * o.nv + col)];
         }
         v[b][k] = u;
       }
@@ -777,7 +780,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
 #else
           const float pg = Pg[b][k];
 #endif
-          v[b][k] = fft::cmul(S[b][k], tf_value(o, pg, c, d, j + k * FL::CNT, col0 + col));
+          v[b][k] = fft::cmul(S[b][k], tf_value<BAND>(o, pg, c, d, j + k * FL::CNT, col0 + col));
         }
       }
       Iv::template first<-1, CG, NT>(buf, v);
@@ -792,11 +795,10 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
       if (!IL::live(b)) continue;
       int j, col;
       IL::item(b, j, col);
-      float2* db = dst + (ptrdiff_t)(j - o.or0) * o.nv + col;
 #pragma unroll
       for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
         const int rr = j + k * IL::CNT - o.or0;
-        if ((unsigned)rr < (unsigned)o.oh) db[(size_t)(k * IL::CNT) * o.nv] = u[b][k];
+        if ((unsigned)rr < (unsigned)o.oh) dst[(unsigned)(rr * o.nv + col)] = u[b][k];
       }
     }
   }
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
 
 // A2 (see k_a2_cols) with a compile-time plan.  The per-plane forward transforms read
 // their twiddles from the forward pipe's stage tables staged in LDS.  grid (8 * per, C)
-template <class P>
+template <class P, bool BAND>
 __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __restrict__ Z, Optics o,
                                                               fft::Plan plan,
                                                               const float2* __restrict__ tabs,
@@ -851,13 +853,12 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
         if (!F1::live(b)) continue;
         int j, col;
         F1::item(b, j, col);
-        const float2* sb = src + (ptrdiff_t)(j - o.or0) * o.nv + col;
 #pragma unroll
         for (int k = 0; k < Fw::FIRST; ++k) {
           float2 u = make_float2(0.f, 0.f);
           if (in_mid_half<Fw>(k)) {
             const int rr = j + k * F1::CNT - o.or0;
-            if ((unsigned)rr < (unsigned)o.oh) u = sb[(size_t)(k * F1::CNT) * o.nv];
+            if ((unsigned)rr < (unsigned)o.oh) u = src[(unsigned)(rr * o.nv + col)];
           }
           v[b][k] = u;
         }
@@ -880,7 +881,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
 #else
         const float pg = Pg[b][k];
 #endif
-        const float2 h = tf_value(o, pg, c, d, j + k * FL::CNT, col0 + col);
+        const float2 h = tf_value<BAND>(o, pg, c, d, j + k * FL::CNT, col0 + col);
         A[b][k].x += v[b][k].x * h.x + v[b][k].y * h.y;   // v * conj(h)
         A[b][k].y += v[b][k].y * h.x - v[b][k].x * h.y;
       }
@@ -898,11 +899,10 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
     if (!IL::live(b)) continue;
     int j, col;
     IL::item(b, j, col);
-    float2* db = dst + (ptrdiff_t)(j - o.gr0) * o.nv + col;
 #pragma unroll
     for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
       const int rr = j + k * IL::CNT - o.gr0;
-      if ((unsigned)rr < (unsigned)o.H) db[(size_t)(k * IL::CNT) * o.nv] = u[b][k];
+      if ((unsigned)rr < (unsigned)o.H) dst[(unsigned)(rr * o.nv + col)] = u[b][k];
     }
   }
 }
@@ -951,8 +951,12 @@ using RowP8320 = RowPlan<640, Pipe<8320, Rad<16, 8, 5, 13>, Lays<Ly<32, 1>, Ly<3
 using RowP1152 = RowPlan<128, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 1>, Ly<32, 4>>>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<64, 1>>>>;
 // compile-time column plans (2816: 1080p, 1152: 256^2, 4992: 2160p)
-using ColP2816 = ColPlan<2, 512, CVQPG_WPE_F, CVQPG_WPE_A, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 2>, Ly0>>,
-                         Pipe<2816, Rad<11, 16, 16>, Lays<Ly0, Ly<32, 2>>>>;
+// 2816: the forward runs 2 columns per 512-thread workgroup (4 waves per SIMD, a small
+// spill), the adjoint one column per 256-thread workgroup (3 waves per SIMD)
+using ColP2816F = ColPlan<2, 512, CVQPG_WPE_F, 1, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 2>, Ly0>>,
+                          Pipe<2816, Rad<11, 16, 16>, Lays<Ly0, Ly<32, 2>>>>;
+using ColP2816A = ColPlan<1, 256, 1, CVQPG_WPE_A, Pipe<2816, Rad<16, 16, 11>, Lays<Ly<32, 1>, Ly<32, 2>>>,
+                          Pipe<2816, Rad<11, 16, 16>, Lays<Ly0, Ly0>>>;
 using ColP1152 = ColPlan<4, 512, 2, 2, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 2>, Ly0>>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<128, 2>>>>;
 using ColP4992 = ColPlan<1, 384, 2, 2, Pipe<4992, Rad<16, 8, 3, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
@@ -1108,19 +1112,29 @@ static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& p
   const int nb = o.nv / P::cg, per = ceil_div(nb, 8);
   const int tab = ADJ ? P::Fwd::TABLE : P::Inv::TABLE;
   const size_t sh = (size_t)(col_lds_head<P>() + tab) * sizeof(float2);
-  if constexpr (ADJ)
-    launch_fx<P>(k_a2_cols_fx<P>, dim3(8 * per, o.C), sh, s, (const float2*)w.Y, o, pr,
-                 (const float2*)w.tw_cs, per, w.X);
-  else
-    launch_fx<P>(k_f2_cols_fx<P>, dim3(8 * per, o.C), sh, s, (const float2*)w.X, o, pr,
-                 (const float2*)w.tw_cs, per, w.Y);
+  const dim3 grid(8 * per, o.C);
+  const float2* tabs = w.tw_cs;
+  if constexpr (ADJ) {
+    if (o.band_active)
+      launch_fx<P>(k_a2_cols_fx<P, true>, grid, sh, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
+    else
+      launch_fx<P>(k_a2_cols_fx<P, false>, grid, sh, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
+  } else {
+    if (o.band_active)
+      launch_fx<P>(k_f2_cols_fx<P, true>, grid, sh, s, (const float2*)w.X, o, pr, tabs, per, w.Y);
+    else
+      launch_fx<P>(k_f2_cols_fx<P, false>, grid, sh, s, (const float2*)w.X, o, pr, tabs, per, w.Y);
+  }
 }
 
 template <int CG, bool ADJ>
 static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
   // compile-time column plans of the standard workloads (2816: 1080p, 4992: 2160p, 1152: 256^2)
   if constexpr (CG == 2) {
-    if (o.nu == 2816) return launch_cols_fx<ColP2816, ADJ>(w, o, pr, s);
+    if (o.nu == 2816) {
+      if constexpr (ADJ) return launch_cols_fx<ColP2816A, ADJ>(w, o, pr, s);
+      else return launch_cols_fx<ColP2816F, ADJ>(w, o, pr, s);
+    }
   }
   if constexpr (CG == 4) {
     if (o.nu == 1152) return launch_cols_fx<ColP1152, ADJ>(w, o, pr, s);
