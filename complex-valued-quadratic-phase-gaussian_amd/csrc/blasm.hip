@@ -168,6 +168,21 @@ __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int 
   return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
 }
 
+// Row spectra (X, Z: hundreds of MB, read once by the next pass) are written with
+// non-temporal stores; whole rows are written, so every line is complete.  The column
+// passes' 8- and 16-byte pieces of a row keep ordinary stores (merged in L2).
+#ifndef CVQPG_NT_STORES
+#define CVQPG_NT_STORES 1
+#endif
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+#if CVQPG_NT_STORES
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+#else
+  *p = v;
+#endif
+}
+
 // ---------------------------------------------------------------------------------
 // glue: u = clamp(|p|, 0, 1) e^{i angle(p)}  (== p for |p| <= 1, p/|p| otherwise)
 // ---------------------------------------------------------------------------------
@@ -538,7 +553,7 @@ __global__ __launch_bounds__(P::nt) void k_f1_rows_fx(const float2* __restrict__
     int j, col;
     GL::item(b, j, col);
 #pragma unroll
-    for (int k = 0; k < Fw::LAST; ++k) dst[j + k * GL::CNT] = v[b][k];
+    for (int k = 0; k < Fw::LAST; ++k) stream_store(dst + j + k * GL::CNT, v[b][k]);
   }
 }
 
@@ -640,7 +655,7 @@ __global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ 
     int j, col;
     GL::item(b, j, col);
 #pragma unroll
-    for (int k = 0; k < Fw::LAST; ++k) dst[j + k * GL::CNT] = v[b][k];
+    for (int k = 0; k < Fw::LAST; ++k) stream_store(dst + j + k * GL::CNT, v[b][k]);
   }
 }
 
