@@ -58,6 +58,11 @@ SIGNATURES = {
                                            _p, _p]),
     "cvqpg_adan_step": (_c_int, [_p, _p, _p, _p, _p, _c_int, _p, _c_f32, _c_f32, _c_int, _c_f32,
                                  _c_f32, _c_f32, _c_f32, _c_f32, _c_int, _c_int, _p, _p]),
+    "cvqpg_soft_window_pad": (_c_int, [_p] + [_c_int] * 5 + [_p, _p]),
+    "cvqpg_dpac_encode": (_c_int, [_p] + [_c_int] * 3 + [_p, _p]),
+    "cvqpg_metrics_workspace_size": (_c_size, [_c_int] * 4),
+    "cvqpg_plane_metrics": (_c_int, [_p, _c_int, _c_int, _p] + [_c_int] * 4 + [_p, _c_size, _p,
+                                                                             _p, _p]),
 }
 
 _LIB = None

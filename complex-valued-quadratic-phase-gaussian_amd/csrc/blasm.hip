@@ -27,6 +27,7 @@
 #include "common.h"
 #include "cvqpg.h"
 #include "fft.h"
+#include "window.h"
 
 namespace cvqpg {
 
@@ -181,41 +182,6 @@ __device__ __forceinline__ void stream_store(float2* p, float2 v) {
 #else
   *p = v;
 #endif
-}
-
-// ---------------------------------------------------------------------------------
-// glue: u = clamp(|p|, 0, 1) e^{i angle(p)}  (== p for |p| <= 1, p/|p| otherwise)
-// ---------------------------------------------------------------------------------
-__device__ __forceinline__ float2 glue(float2 p) {
-  const float a2 = p.x * p.x + p.y * p.y;
-  if (a2 > 1.0f) {
-    const float inv = rsqrtf(a2);
-    return make_float2(p.x * inv, p.y * inv);
-  }
-  return p;
-}
-
-// adjoint of glue for gradient g = dL/dRe u + i dL/dIm u (torch's convention, with the
-// gradient of abs and angle defined as 0 at the origin)
-__device__ __forceinline__ float2 glue_adjoint(float2 h, float2 g) {
-  const float a2 = h.x * h.x + h.y * h.y;
-  if (a2 == 0.0f) return make_float2(0.f, 0.f);
-  if (a2 <= 1.0f) return g;
-  const float a = sqrtf(a2);
-  const float hx = h.x / a, hy = h.y / a;
-  const float q = (g.y * hx - g.x * hy) / a;  // Im(g conj(hh)) / |h|
-  return make_float2(-q * hy, q * hx);         // q * i * hh
-}
-
-__device__ __forceinline__ float taper(int t, int total, int inner_start, int inner_size) {
-  if (t < inner_start)
-    return 0.5f * (1.0f - cosf(3.14159265358979f * (float)(t + 1) / (float)inner_start));
-  const int rs = inner_start + inner_size;
-  if (t >= rs) {
-    const int n_right = total - rs;
-    return 0.5f * (1.0f + cosf(3.14159265358979f * (float)(t - rs + 1) / (float)n_right));
-  }
-  return 1.0f;
 }
 
 // ---------------------------------------------------------------------------------
@@ -959,8 +925,10 @@ static size_t row_lds_bytes(const Optics& o) {
 }
 
 // compile-time row plans of the standard workloads (4480: 1080p, 8320: 2160p, 1152: 256^2)
-using RowP4480 = RowPlan<320, Pipe<4480, Rad<16, 8, 5, 7>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
-                         Pipe<4480, Rad<7, 5, 8, 16>, Lays<Ly0, Ly<32, 4>, Ly0>>>;
+// 4480: layouts with at most 1/16 padding, so that four workgroups (39 KB of LDS each)
+// fit on a CU
+using RowP4480 = RowPlan<320, Pipe<4480, Rad<16, 8, 5, 7>, Lays<Ly<32, 1>, Ly<32, 2>, Ly0>>,
+                         Pipe<4480, Rad<7, 5, 8, 16>, Lays<Ly0, Ly<64, 1>, Ly0>>>;
 using RowP8320 = RowPlan<640, Pipe<8320, Rad<16, 8, 5, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
                          Pipe<8320, Rad<13, 5, 8, 16>, Lays<Ly0, Ly<32, 4>, Ly0>>>;
 using RowP1152 = RowPlan<128, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 1>, Ly<32, 4>>>,

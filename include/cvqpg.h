@@ -244,6 +244,33 @@ int cvqpg_adan_step(float *const *params, const int64_t *sizes, const float *gra
                     float beta3, float eps, float max_norm, int curv_warmup, int curv_enabled,
                     int32_t *iteration, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Export and evaluation.                                                   */
+/* ------------------------------------------------------------------------ */
+/* soft_window_pad (model_2d_gaussian_rgb.py:186-241) of a complex64 field [C][H][W] into
+ * out [C][pad_height][pad_width]: replicate padding times the Tukey guard-band window. */
+int cvqpg_soft_window_pad(const float *field, int channels, int height, int width,
+                          int pad_height, int pad_width, float *out, void *stream);
+
+/* DPAC phase-only encoding to 8-bit levels, replacing export_poh.py:110-119,173-178:
+ * phase = angle(h) mod 2pi, amplitude = clamp(|h|, 0, 1), phase -+ arccos(amplitude) on
+ * the checkerboard, mod 2pi, level = round(. / 2pi * 256) mod 256 (half to even).
+ * field complex64 [C][pad_height][pad_width]; levels uint8 of the same shape. */
+int cvqpg_dpac_encode(const float *field, int channels, int pad_height, int pad_width,
+                      uint8_t *levels, void *stream);
+
+/* Per-plane PSNR and SSIM of reconstructions against the targets, replacing the skimage
+ * calls of train_2d_gaussian_rgb.py:98-112 (data_range 1; SSIM: 7x7 uniform window,
+ * sample covariance, K = (0.01, 0.03), mean over the interior and over the channels).
+ * recon [planes][channels][recon_height][recon_width] (clamped to [0, 1] and centre-
+ * cropped to height x width as odak crop_center does), target [planes][channels][height]
+ * [width]; psnr and ssim are device arrays [planes].  Workspace from
+ * cvqpg_metrics_workspace_size. */
+size_t cvqpg_metrics_workspace_size(int planes, int channels, int height, int width);
+int cvqpg_plane_metrics(const float *recon, int recon_height, int recon_width,
+                        const float *target, int planes, int channels, int height, int width,
+                        void *ws, size_t ws_bytes, float *psnr, float *ssim, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

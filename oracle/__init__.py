@@ -25,6 +25,9 @@ Modules and the reference code they restate:
                    202-290`` and odak's BLASM kernel it calls).
 * ``loss``      -- the three-term masked L2, SSIM term and target synthesis
                    (``utils/propagator.py:361-449``, ``utils/data_utils.py:22-82``).
+* ``export``    -- DPAC phase-only encoding with 8-bit quantisation and the per-plane
+                   PSNR / SSIM evaluation (``export_poh.py:110-119,173-178``,
+                   ``train_2d_gaussian_rgb.py:98-112``).
 * ``adan``      -- Adan optimiser step, nan_to_num, global clip and the means
                    cosine schedule (``utils/optimizer.py:105-262``,
                    ``train_2d_gaussian_rgb.py:58-86,205-211``).
