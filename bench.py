@@ -29,9 +29,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", default="fused", choices=["fused", "autograd"],
+    ap.add_argument("--path", default="fused", choices=["fused", "autograd", "torch"],
                     help="fused: all-HIP graph-captured step; autograd: reference-structured "
-                         "step (HIP raster, PyTorch elsewhere)")
+                         "step (HIP raster, PyTorch elsewhere); torch: the reference's PyTorch "
+                         "path on the GPU (3-sigma / 64-px tile renderer, torch.fft BLASM, "
+                         "torch loss and Adan), the denominator of the speed-up target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -144,12 +146,14 @@ def main():
     W, H, N = spec["W"], spec["H"], spec["N"]
     C = len(spec["wavelengths"])
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
-    wl = train.Workload(args.workload, device=dev, seed=0)
+    wl = train.Workload(args.workload, device=dev, seed=0,
+                        renderer="torch" if args.path == "torch" else "hip")
     pH, pW = synthetic.pad_size_for(W, H)
     f_fwd, f_bwd = raster_flops_per_pair(C)
-    if args.path == "autograd":
+    if args.path in ("autograd", "torch"):
         trainer = train.AutogradTrainer(wl, backend="torch")
-        path = "autograd: HIP raster + torch.fft BLASM + torch loss/Adan"
+        path = ("autograd: HIP raster + torch.fft BLASM + torch loss/Adan" if args.path == "autograd"
+                else "torch: PyTorch tile renderer + torch.fft BLASM + torch loss/Adan")
         for _ in range(args.warmup):
             trainer.step()
         torch.cuda.synchronize()
@@ -186,14 +190,16 @@ def main():
         P = trainer.last_pairs
     ms = 1e3 * dt / args.steps
     pairs = P * 256
-    achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
     traffic = measured_traffic() if args.workload == WORKLOAD else None
-    roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)", "bound": "mfma",
-            "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,
-            "pairs": pairs, "flop_per_pair": f_bwd}
-    if traffic is not None:
-        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    roof = None   # the PyTorch path has no HIP raster kernel to report
+    if k_ms:
+        achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)", "bound": "mfma",
+                "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,
+                "pairs": pairs, "flop_per_pair": f_bwd}
+        if traffic is not None:
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if args.calib_copy > 0:
         n = args.calib_copy * (1 << 20) // 4
         src = torch.ones(n, device=dev)
