@@ -218,6 +218,7 @@ def main():
            "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
                       "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
+                      "peak_hbm_gb": torch.cuda.max_memory_allocated(dev) / 1e9,
                       "stage_ms_eager": stages},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
