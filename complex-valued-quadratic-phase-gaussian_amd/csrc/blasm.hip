@@ -933,6 +933,13 @@ using RowP8320 = RowPlan<640, Pipe<8320, Rad<16, 8, 5, 13>, Lays<Ly<32, 1>, Ly<3
                          Pipe<8320, Rad<13, 5, 8, 16>, Lays<Ly0, Ly<32, 4>, Ly0>>>;
 using RowP1152 = RowPlan<128, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 1>, Ly<32, 4>>>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<64, 1>>>>;
+// row lengths of the batched configuration's frames (512^2, 1024x768, 1280x720)
+using RowP1664 = RowPlan<128, Pipe<1664, Rad<16, 8, 13>, Lays<Ly<32, 1>, Ly<32, 2>>>,
+                         Pipe<1664, Rad<13, 8, 16>, Lays<Ly0, Ly0>>>;
+using RowP2688 = RowPlan<192, Pipe<2688, Rad<16, 8, 3, 7>, Lays<Ly<32, 1>, Ly<32, 2>, Ly0>>,
+                         Pipe<2688, Rad<7, 3, 8, 16>, Lays<Ly0, Ly0, Ly0>>>;
+using RowP3200 = RowPlan<256, Pipe<3200, Rad<16, 8, 5, 5>, Lays<Ly<32, 1>, Ly<32, 2>, Ly0>>,
+                         Pipe<3200, Rad<5, 5, 8, 16>, Lays<Ly0, Ly0, Ly0>>>;
 // compile-time column plans (2816: 1080p, 1152: 256^2, 4992: 2160p)
 // 2816: the forward runs 2 columns per 512-thread workgroup (4 waves per SIMD, a small
 // spill), the adjoint one column per 256-thread workgroup (3 waves per SIMD)
@@ -944,6 +951,13 @@ using ColP1152 = ColPlan<4, 512, 2, 2, Pipe<1152, Rad<16, 8, 9>, Lays<Ly<32, 2>,
                          Pipe<1152, Rad<9, 8, 16>, Lays<Ly0, Ly<128, 2>>>>;
 using ColP4992 = ColPlan<1, 384, 2, 2, Pipe<4992, Rad<16, 8, 3, 13>, Lays<Ly<32, 1>, Ly<32, 4>, Ly0>>,
                          Pipe<4992, Rad<13, 3, 8, 16>, Lays<Ly0, Ly0, Ly0>>>;
+// column lengths of the batched configuration's frames (512^2, 1024x768, 1280x720)
+using ColP1664 = ColPlan<2, 256, 1, 1, Pipe<1664, Rad<16, 8, 13>, Lays<Ly<32, 2>, Ly0>>,
+                         Pipe<1664, Rad<13, 8, 16>, Lays<Ly0, Ly0>>>;
+using ColP2176 = ColPlan<2, 320, 1, 1, Pipe<2176, Rad<16, 8, 17>, Lays<Ly<32, 2>, Ly0>>,
+                         Pipe<2176, Rad<17, 8, 16>, Lays<Ly0, Ly0>>>;
+using ColP2048 = ColPlan<2, 256, 1, 1, Pipe<2048, Rad<16, 16, 8>, Lays<Ly<32, 2>, Ly0>>,
+                         Pipe<2048, Rad<8, 16, 16>, Lays<Ly<32, 2>, Ly<32, 2>>>>;
 
 template <class P>
 static size_t row_fx_lds_bytes() {
@@ -955,6 +969,9 @@ static bool with_row_plan(int n, Fn&& fn) {
     case 4480: fn(RowP4480{}); return true;
     case 8320: fn(RowP8320{}); return true;
     case 1152: fn(RowP1152{}); return true;
+    case 1664: fn(RowP1664{}); return true;
+    case 2688: fn(RowP2688{}); return true;
+    case 3200: fn(RowP3200{}); return true;
     default: return false;
   }
 }
@@ -1118,6 +1135,9 @@ static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, 
       if constexpr (ADJ) return launch_cols_fx<ColP2816A, ADJ>(w, o, pr, s);
       else return launch_cols_fx<ColP2816F, ADJ>(w, o, pr, s);
     }
+    if (o.nu == 1664) return launch_cols_fx<ColP1664, ADJ>(w, o, pr, s);
+    if (o.nu == 2176) return launch_cols_fx<ColP2176, ADJ>(w, o, pr, s);
+    if (o.nu == 2048) return launch_cols_fx<ColP2048, ADJ>(w, o, pr, s);
   }
   if constexpr (CG == 4) {
     if (o.nu == 1152) return launch_cols_fx<ColP1152, ADJ>(w, o, pr, s);

@@ -93,10 +93,14 @@ FIXED_CASES = [  # (C, H, W, pH, pW, distances): the guard bands of the standard
     (1, 256, 256, 576, 576, [0.0, 2.5e-3, 5e-3]),      # 1152 rows and columns
     (3, 1080, 1920, 1408, 2240, [0.0, 6e-3]),           # 2816 columns, 4480 rows
     (1, 2160, 3840, 2496, 4160, [5e-3]),                # 4992 columns, 8320 rows
+    (3, 512, 512, 832, 832, [3e-3]),                    # 1664 rows and columns
+    (3, 768, 1024, 1088, 1344, [3e-3]),                 # 2176 columns, 2688 rows
+    (3, 720, 1280, 1024, 1600, [3e-3]),                 # 2048 columns, 3200 rows
 ]
 
 
-@pytest.mark.parametrize("C,H,W,pH,pW,dists", FIXED_CASES, ids=["256sq", "1080p", "2160p"])
+@pytest.mark.parametrize("C,H,W,pH,pW,dists", FIXED_CASES,
+                         ids=["256sq", "1080p", "2160p", "512sq", "1024x768", "720p"])
 def test_fixed_plan_sizes_match_fp64(pkg, C, H, W, pH, pW, dists):
     """Full-size forward and adjoint on the compile-time transform plans."""
     test_forward_and_adjoint_match_fp64(pkg, C, H, W, pH, pW, dists, "odak")
