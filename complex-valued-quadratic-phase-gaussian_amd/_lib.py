@@ -39,6 +39,10 @@ SIGNATURES = {
                                              _p, _p, _p, _p]),
     "cvqpg_render_backward_tiles": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _c_int, _p, _c_size,
                                              _c_i64, _p, _p, _p]),
+    "cvqpg_raster_batch_table_size": (_c_size, [_c_int]),
+    "cvqpg_raster_batch_table": (_c_int, [_p, _c_int, _p, _c_size]),
+    "cvqpg_render_forward_batch": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
+    "cvqpg_render_backward_batch": (_c_int, [_p, _c_int, _c_int, _c_int, _p]),
     "cvqpg_render_backward_reduce": (_c_int, [_p, _p, _p, _c_int, _c_int, _c_f32, _c_int, _c_int,
                                               _p, _c_size, _c_i64] + [_p] * 7 + [_p]),
     "cvqpg_blasm_workspace_size": (_c_size, [_c_int] * 7),
@@ -85,6 +89,14 @@ def lib():
             fn.argtypes = args
         _LIB = handle
     return _LIB
+
+
+class RasterScene(ctypes.Structure):
+    """cvqpg_raster_scene of include/cvqpg.h."""
+    _fields_ = [("n", _c_int), ("width", _c_int), ("height", _c_int), ("channels", _c_int),
+                ("ws", _p), ("ws_bytes", _c_size), ("capacity", _c_i64),
+                ("point_list", _p), ("ranges", _p), ("field", _p), ("grad_field", _p),
+                ("k", _p)]
 
 
 def loaded_path():

@@ -25,6 +25,8 @@
 //     cross-lane reduction; one 64-byte partial record per (tile, Gaussian) key
 //     is written at the key's emission slot and a per-Gaussian pass sums the
 //     contiguous slots (deterministic, no atomics).
+#include <string.h>
+
 #include "common.h"
 #include "cvqpg.h"
 
@@ -389,14 +391,15 @@ __global__ void k_pack(const float2* __restrict__ means, const float* __restrict
 // raster forward
 // ------------------------------------------------------------------------------
 // LDS record per Gaussian (16 floats): mx my a00 a01 | a11 u0 w0 amp0 | u1 w1 amp1 u2 | w2 amp2 - -
+// One 16x16 tile of one hologram; sg is the workgroup's 16 KiB staging buffer.  Every
+// write to sg follows a workgroup barrier inside this function, so consecutive calls by
+// one workgroup need no barrier between them.
 template <int C>
-__global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ranges,
-                                                    const int32_t* __restrict__ point_list,
-                                                    const float4* __restrict__ packed,
-                                                    const uint32_t* __restrict__ misc, int W,
-                                                    int H, int gx, float2* __restrict__ out) {
-  __shared__ float4 sg[256 * 4];
-  const int tile = blockIdx.x;
+__device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict__ ranges,
+                                                const int32_t* __restrict__ point_list,
+                                                const float4* __restrict__ packed,
+                                                const uint32_t* __restrict__ misc, int W, int H,
+                                                int gx, float2* __restrict__ out, float4* sg) {
   const int tx = tile % gx, ty = tile / gx;
   const int t = threadIdx.x;
   const int px = tx * TILE + (t & 15);
@@ -464,21 +467,33 @@ __global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ran
   }
 }
 
+template <int C>
+__global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ranges,
+                                                    const int32_t* __restrict__ point_list,
+                                                    const float4* __restrict__ packed,
+                                                    const uint32_t* __restrict__ misc, int W,
+                                                    int H, int gx, float2* __restrict__ out) {
+  __shared__ float4 sg[256 * 4];
+  render_fwd_tile<C>(blockIdx.x, ranges, point_list, packed, misc, W, H, gx, out, sg);
+}
+
 // ------------------------------------------------------------------------------
 // raster adjoint: one Gaussian per lane, pixels as wave-uniform broadcasts
 // ------------------------------------------------------------------------------
 // partial record (16 floats): dcol[C], dph'[C], dop, K', di00, di01', di11, dmx', dmy'
+// One tile of one hologram; sgr is the workgroup's [C][256] pixel-gradient buffer.  The
+// first write to sgr is not preceded by a barrier here: a workgroup that calls this for
+// several tiles places one between the calls.
 template <int C>
-__global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ranges,
-                                                    const int32_t* __restrict__ point_list,
-                                                    const float4* __restrict__ packed,
-                                                    const int4* __restrict__ rect,
-                                                    const float2* __restrict__ grad,
-                                                    const float* __restrict__ kvec,
-                                                    const uint32_t* __restrict__ misc, int W,
-                                                    int H, int gx, float4* __restrict__ partials) {
-  __shared__ float2 sgr[C][256];
-  const int tile = blockIdx.x;
+__device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict__ ranges,
+                                                const int32_t* __restrict__ point_list,
+                                                const float4* __restrict__ packed,
+                                                const int4* __restrict__ rect,
+                                                const float2* __restrict__ grad,
+                                                const float* __restrict__ kvec,
+                                                const uint32_t* __restrict__ misc, int W, int H,
+                                                int gx, float4* __restrict__ partials,
+                                                float2 (*sgr)[256]) {
   const int tx = tile % gx, ty = tile / gx;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -602,6 +617,120 @@ __global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ran
         dst[i] = make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
     }
   }
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ranges,
+                                                    const int32_t* __restrict__ point_list,
+                                                    const float4* __restrict__ packed,
+                                                    const int4* __restrict__ rect,
+                                                    const float2* __restrict__ grad,
+                                                    const float* __restrict__ kvec,
+                                                    const uint32_t* __restrict__ misc, int W,
+                                                    int H, int gx, float4* __restrict__ partials) {
+  __shared__ float2 sgr[C][256];
+  render_bwd_tile<C>(blockIdx.x, ranges, point_list, packed, rect, grad, kvec, misc, W, H, gx,
+                     partials, sgr);
+}
+
+// ------------------------------------------------------------------------------
+// many holograms in one launch: a persistent grid drains a queue of tiles
+// ------------------------------------------------------------------------------
+// The queue is the concatenation of every scene's tiles in table order.  A workgroup
+// takes `chunk` consecutive tiles per atomic fetch until the queue is empty; the last
+// workgroup to leave resets the fetch and exit counters to zero, so the next launch
+// (stream-ordered after this one) starts from a clean queue with no separate clear.
+constexpr int MAX_BATCH = 64;
+
+struct BatchScene {
+  const int2* ranges;
+  const int32_t* point_list;
+  const float4* packed;
+  const int4* rect;
+  const uint32_t* misc;
+  float2* field;
+  const float2* grad;
+  const float* k;
+  float4* partials;
+  int W, H, gx, tile_begin;
+};
+
+struct BatchHeader {
+  uint32_t fetch[2];  // [0] forward, [1] adjoint
+  uint32_t exits[2];
+  int n_scenes, total_tiles, pad[10];
+};
+static_assert(sizeof(BatchHeader) == 64, "batch header is one 64-byte line");
+
+__device__ __forceinline__ int batch_next(BatchHeader* hdr, int which, int chunk, int* slot) {
+  __syncthreads();  // every lane has read the previous slot value and left the previous tile
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(&hdr->fetch[which], 1u) * chunk;
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+__device__ __forceinline__ void batch_exit(BatchHeader* hdr, int which) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&hdr->exits[which], 1u) == gridDim.x - 1) {
+      atomicExch(&hdr->fetch[which], 0u);
+      atomicExch(&hdr->exits[which], 0u);
+    }
+  }
+}
+
+__device__ __forceinline__ int batch_scene_of(const int* begin, int ns, int item, int s) {
+  while (s + 1 < ns && item >= begin[s + 1]) ++s;
+  return __builtin_amdgcn_readfirstlane(s);
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_render_fwd_batch(BatchHeader* __restrict__ hdr,
+                                                          const BatchScene* __restrict__ sc,
+                                                          int chunk) {
+  __shared__ float4 sg[256 * 4];
+  __shared__ int begin[MAX_BATCH];
+  __shared__ int slot;
+  const int ns = hdr->n_scenes, total = hdr->total_tiles;
+  for (int i = threadIdx.x; i < ns; i += 256) begin[i] = sc[i].tile_begin;
+  for (;;) {
+    const int first = batch_next(hdr, 0, chunk, &slot);
+    if (first >= total) break;
+    const int last = min(first + chunk, total);
+    int s = 0;
+    for (int it = first; it < last; ++it) {
+      s = batch_scene_of(begin, ns, it, s);
+      const BatchScene& S = sc[s];
+      render_fwd_tile<C>(it - S.tile_begin, S.ranges, S.point_list, S.packed, S.misc, S.W, S.H,
+                         S.gx, S.field, sg);
+    }
+  }
+  batch_exit(hdr, 0);
+}
+
+template <int C>
+__global__ __launch_bounds__(256, 5) void k_render_bwd_batch(BatchHeader* __restrict__ hdr,
+                                                          const BatchScene* __restrict__ sc,
+                                                          int chunk) {
+  __shared__ float2 sgr[C][256];
+  __shared__ int begin[MAX_BATCH];
+  __shared__ int slot;
+  const int ns = hdr->n_scenes, total = hdr->total_tiles;
+  for (int i = threadIdx.x; i < ns; i += 256) begin[i] = sc[i].tile_begin;
+  for (;;) {
+    const int first = batch_next(hdr, 1, chunk, &slot);
+    if (first >= total) break;
+    const int last = min(first + chunk, total);
+    int s = 0;
+    for (int it = first; it < last; ++it) {
+      if (it > first) __syncthreads();  // the previous tile's waves are done with sgr
+      s = batch_scene_of(begin, ns, it, s);
+      const BatchScene& S = sc[s];
+      render_bwd_tile<C>(it - S.tile_begin, S.ranges, S.point_list, S.packed, S.rect, S.grad,
+                         S.k, S.misc, S.W, S.H, S.gx, S.partials, sgr);
+    }
+  }
+  batch_exit(hdr, 1);
 }
 
 // per-Gaussian sum of the partial records over its contiguous emission slots
@@ -1045,6 +1174,91 @@ extern "C" int cvqpg_render_backward_tiles(const float* grad_field, const float*
                        reinterpret_cast<const float4*>(w.packed), w.rect,
                        reinterpret_cast<const float2*>(grad_field), k, w.misc, width, height, gx,
                        reinterpret_cast<float4*>(w.partials));
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" size_t cvqpg_raster_batch_table_size(int n_scenes) {
+  return sizeof(BatchHeader) + (size_t)(n_scenes > 0 ? n_scenes : 0) * sizeof(BatchScene);
+}
+
+extern "C" int cvqpg_raster_batch_table(const cvqpg_raster_scene* scenes, int n_scenes,
+                                        void* table_host, size_t table_bytes) {
+  CVQPG_CHECK_ARG(n_scenes >= 1 && n_scenes <= MAX_BATCH, "n_scenes must be in [1, %d] (got %d)",
+                  MAX_BATCH, n_scenes);
+  CVQPG_CHECK_ARG(scenes && table_host, "NULL scenes/table");
+  CVQPG_CHECK_ARG(table_bytes >= cvqpg_raster_batch_table_size(n_scenes),
+                  "table too small: %zu < %zu bytes", table_bytes,
+                  cvqpg_raster_batch_table_size(n_scenes));
+  BatchHeader* hdr = static_cast<BatchHeader*>(table_host);
+  BatchScene* sc = reinterpret_cast<BatchScene*>(hdr + 1);
+  memset(table_host, 0, cvqpg_raster_batch_table_size(n_scenes));
+  int64_t tiles = 0;
+  for (int i = 0; i < n_scenes; ++i) {
+    const cvqpg_raster_scene& q = scenes[i];
+    if (int rc = check_common(q.n, q.channels, q.width, q.height, q.ws, q.ws_bytes, q.capacity))
+      return rc;
+    CVQPG_CHECK_ARG(q.channels == scenes[0].channels, "scene %d has %d channels, scene 0 has %d",
+                    i, q.channels, scenes[0].channels);
+    CVQPG_CHECK_ARG(q.point_list && q.ranges && q.field, "scene %d: NULL point_list/ranges/field",
+                    i);
+    RasterWS w = carve(q.ws, q.n, q.capacity);
+    const int gx = ceil_div(q.width, TILE), gy = ceil_div(q.height, TILE);
+    BatchScene& b = sc[i];
+    b.ranges = reinterpret_cast<const int2*>(q.ranges);
+    b.point_list = q.point_list;
+    b.packed = reinterpret_cast<const float4*>(w.packed);
+    b.rect = w.rect;
+    b.misc = w.misc;
+    b.field = reinterpret_cast<float2*>(q.field);
+    b.grad = reinterpret_cast<const float2*>(q.grad_field);
+    b.k = q.k;
+    b.partials = reinterpret_cast<float4*>(w.partials);
+    b.W = q.width;
+    b.H = q.height;
+    b.gx = gx;
+    b.tile_begin = (int)tiles;
+    tiles += (int64_t)gx * gy;
+    CVQPG_CHECK_ARG(tiles < ((int64_t)1 << 30), "too many tiles in one batch");
+  }
+  hdr->n_scenes = n_scenes;
+  hdr->total_tiles = (int)tiles;
+  return 0;
+}
+
+static int check_batch_launch(void* table, int channels, int grid, int chunk) {
+  CVQPG_CHECK_ARG(table != nullptr, "table is NULL");
+  CVQPG_CHECK_ARG(channels == 1 || channels == 3,
+                  "qpf rasterizer supports 1 or 3 channels, got %d", channels);
+  CVQPG_CHECK_ARG(grid >= 1 && grid <= 65536, "grid must be in [1, 65536] (got %d)", grid);
+  CVQPG_CHECK_ARG(chunk >= 1 && chunk <= 64, "chunk must be in [1, 64] (got %d)", chunk);
+  return 0;
+}
+
+extern "C" int cvqpg_render_forward_batch(void* table, int channels, int grid, int chunk,
+                                          void* stream) {
+  if (int rc = check_batch_launch(table, channels, grid, chunk)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  BatchHeader* hdr = static_cast<BatchHeader*>(table);
+  const BatchScene* sc = reinterpret_cast<const BatchScene*>(hdr + 1);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_render_fwd_batch<3>, dim3(grid), dim3(256), 0, s, hdr, sc, chunk);
+  else
+    hipLaunchKernelGGL(k_render_fwd_batch<1>, dim3(grid), dim3(256), 0, s, hdr, sc, chunk);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_render_backward_batch(void* table, int channels, int grid, int chunk,
+                                           void* stream) {
+  if (int rc = check_batch_launch(table, channels, grid, chunk)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  BatchHeader* hdr = static_cast<BatchHeader*>(table);
+  const BatchScene* sc = reinterpret_cast<const BatchScene*>(hdr + 1);
+  if (channels == 3)
+    hipLaunchKernelGGL(k_render_bwd_batch<3>, dim3(grid), dim3(256), 0, s, hdr, sc, chunk);
+  else
+    hipLaunchKernelGGL(k_render_bwd_batch<1>, dim3(grid), dim3(256), 0, s, hdr, sc, chunk);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }

@@ -36,6 +36,28 @@ WORKLOADS = {
 }
 
 
+BATCH_RESOLUTIONS = [(512, 512), (1024, 768), (1280, 720), (1920, 1080)]
+
+
+def batch_specs(n_scenes=16, seed=0, resolutions=None, density=0.07, n_range=(20_000, 150_000),
+                num_planes=3, z_range=(0.0, 6e-3)):
+    """Specs of a batch of independent RGB scenes of uneven size: resolution drawn from
+    ``resolutions``, N = density x pixels clamped to ``n_range``, ``num_planes`` plane
+    distances drawn uniformly in ``z_range`` (sorted) per scene.  Scene i uses seed
+    ``seed + i`` for its Gaussians, target and depth."""
+    rng = np.random.default_rng(seed + 7_000_001)
+    res = list(resolutions or BATCH_RESOLUTIONS)
+    out = []
+    for i in range(n_scenes):
+        W, H = res[int(rng.integers(len(res)))]
+        N = int(min(max(round(density * W * H), n_range[0]), n_range[1]))
+        z = np.sort(rng.uniform(z_range[0], z_range[1], num_planes))
+        out.append(dict(W=W, H=H, N=N, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
+                        distances=[float(np.float32(v)) for v in z], kappa=0.02,
+                        seed=seed + i))
+    return out
+
+
 def pad_size_for(W, H):
     """Guard band >= 144 px per side rounded up to 64 (train_2d_gaussian_rgb.py:340-343)."""
     return [((H + 2 * 144 + 63) // 64) * 64, ((W + 2 * 144 + 63) // 64) * 64]

@@ -131,6 +131,38 @@ int cvqpg_qpf_render_forward(const float *means, const float *inv_cov, const flo
                              int32_t *point_list, int32_t *ranges, int32_t *num_pairs,
                              void *stream);
 
+/* Many independent holograms in one launch.  The reference renders one scene per
+ * qpf_render call (cuda/rasterizer.py:62-71 -> cuda/raster.cu:30-143); these entry
+ * points run the same per-tile forward (cuda/forward.cu:62-162) and adjoint
+ * (cuda/backward.cu:22-148) for several scenes from one persistent grid that drains a
+ * queue of all scenes' tiles.  Each scene is described as it would be for
+ * cvqpg_render_forward_packed / cvqpg_render_backward_tiles (its own workspace, key
+ * buffers and field); all scenes share the channel count. */
+typedef struct cvqpg_raster_scene {
+  int n, width, height, channels;
+  void *ws;                 /* raster workspace of this scene (after cvqpg_prologue_forward
+                               or cvqpg_raster_pack and cvqpg_bin_count / cvqpg_bin_sort) */
+  size_t ws_bytes;
+  int64_t capacity;
+  const int32_t *point_list, *ranges;
+  float *field;             /* forward output, complex64 [C][H][W] */
+  const float *grad_field;  /* adjoint input, complex64 [C][H][W] (may be NULL for forward only) */
+  const float *k;           /* [C] wave numbers (adjoint) */
+} cvqpg_raster_scene;
+
+/* Bytes of the device table for n_scenes scenes (at most 64). */
+size_t cvqpg_raster_batch_table_size(int n_scenes);
+/* Host-only: validates the scenes and writes the table image into table_host (the
+ * caller copies it to a device buffer of the same size, once; the queue counters it
+ * holds start at zero and are left at zero by every completed launch).  Scenes are
+ * queued in the given order (put the heaviest first). */
+int cvqpg_raster_batch_table(const cvqpg_raster_scene *scenes, int n_scenes, void *table_host,
+                             size_t table_bytes);
+/* Forward / adjoint over every scene of a device table; grid = persistent workgroups,
+ * chunk = tiles taken per queue fetch.  Launches that share one table must not overlap. */
+int cvqpg_render_forward_batch(void *table, int channels, int grid, int chunk, void *stream);
+int cvqpg_render_backward_batch(void *table, int channels, int grid, int chunk, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Band-limited angular spectrum propagation.                                */
 /* The reference has no native code here: these replace the PyTorch/odak    */

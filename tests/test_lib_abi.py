@@ -58,3 +58,57 @@ def test_workspace_size_and_argument_errors(pkg):
     assert b"1 or 3 channels" in L.cvqpg_last_error()
     rc = L.cvqpg_bin_count(None, None, 10, 64, 64, None, 0, 0, None, None)
     assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+
+
+def test_batch_table_layout_and_errors(pkg):
+    """cvqpg_raster_batch_table is host-only: scenes in queue order, tile offsets, and the
+    argument checks (no GPU needed, fake device addresses)."""
+    import numpy as np
+    from cvqpg_amd import _lib
+    L = _lib.lib()
+    sizes = [(1920, 1080, 5000), (512, 512, 100), (100, 36, 7)]
+    scenes = (_lib.RasterScene * 3)()
+    for i, (W, H, n) in enumerate(sizes):
+        s = scenes[i]
+        s.n, s.width, s.height, s.channels = n, W, H, 3
+        s.capacity = 4 * n
+        s.ws = 0x10_0000_0000 * (i + 1)
+        s.ws_bytes = L.cvqpg_raster_workspace_size(n, 3, W, H, s.capacity)
+        s.point_list, s.ranges, s.field = s.ws + 0x1000_0000, s.ws + 0x2000_0000, s.ws + 0x3000_0000
+        s.grad_field, s.k = s.ws + 0x4000_0000, s.ws + 0x5000_0000
+    nb = L.cvqpg_raster_batch_table_size(3)
+    assert nb == 64 + 3 * 88
+    buf = (ctypes.c_uint8 * nb)()
+    _lib.check(L.cvqpg_raster_batch_table(scenes, 3, buf, nb))
+    raw = bytes(buf)
+    hdr = np.frombuffer(raw[:64], dtype=np.int32)
+    tiles = [((W + 15) // 16) * ((H + 15) // 16) for W, H, _ in sizes]
+    assert list(hdr[:4]) == [0, 0, 0, 0]
+    assert hdr[4] == 3 and hdr[5] == sum(tiles)
+    begin = 0
+    for i, (W, H, n) in enumerate(sizes):
+        rec = raw[64 + 88 * i: 64 + 88 * (i + 1)]
+        ptrs = np.frombuffer(rec[:72], dtype=np.uint64)
+        ints = np.frombuffer(rec[72:], dtype=np.int32)
+        s = scenes[i]
+        # ranges, point_list, packed (= workspace base), rect, misc, field, grad, k, partials
+        assert ptrs[0] == s.ranges and ptrs[1] == s.point_list and ptrs[2] == s.ws
+        assert s.ws < ptrs[3] < ptrs[4] < ptrs[8] < s.ws + s.ws_bytes
+        assert ptrs[5] == s.field and ptrs[6] == s.grad_field and ptrs[7] == s.k
+        assert list(ints) == [W, H, (W + 15) // 16, begin]
+        begin += tiles[i]
+    # errors: mixed channels, workspace too small, table too small, scene count
+    scenes[2].channels = 1
+    assert L.cvqpg_raster_batch_table(scenes, 3, buf, nb) == 1
+    assert b"channels" in L.cvqpg_last_error()
+    scenes[2].channels = 3
+    scenes[1].ws_bytes = 100
+    assert L.cvqpg_raster_batch_table(scenes, 3, buf, nb) == 1
+    assert b"workspace too small" in L.cvqpg_last_error()
+    scenes[1].ws_bytes = L.cvqpg_raster_workspace_size(100, 3, 512, 512, 400)
+    assert L.cvqpg_raster_batch_table(scenes, 3, buf, nb - 1) == 1
+    assert L.cvqpg_raster_batch_table(scenes, 0, buf, nb) == 1
+    assert L.cvqpg_raster_batch_table(scenes, 65, buf, nb) == 1
+    assert L.cvqpg_render_forward_batch(None, 3, 256, 1, None) == 1
+    assert L.cvqpg_render_backward_batch(buf, 2, 256, 1, None) == 1
+    assert L.cvqpg_render_forward_batch(buf, 3, 256, 0, None) == 1
