@@ -35,7 +35,9 @@ def parse():
                          "path on the GPU (3-sigma / 64-px tile renderer, torch.fft BLASM, "
                          "torch loss and Adan), the denominator of the speed-up target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--workload", default=WORKLOAD,
+                    help="rgb1080p_150k (the metric's workload), gray256_10k, rgb4k_600k, or "
+                         "batch16 (16 uneven scenes, batched vs sequential)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB after the timed steps (a "
                          "plain copy kernel of known size for scaling HBM byte counters)")
@@ -129,6 +131,61 @@ def stage_report(trainer, steps=3):
     return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in ev.items()}
 
 
+BATCH_METRIC = "aggregate train step/s, 16 uneven RGB scenes in one captured step"
+
+
+def bench_batch(args, dev):
+    """16 independent scenes (512^2 / 1024x768 / 1280x720 / 1920x1080, N = 0.07 x pixels
+    clamped to 20k-150k, 3 planes at seeded z in 0-6 mm): aggregate step/s of the
+    batched graph against the same 16 scenes stepped one after another, each from its
+    own captured graph."""
+    import torch
+    from cvqpg_amd import batch, synthetic
+    specs = synthetic.batch_specs(16, seed=0)
+    print("[bench] building 16 scenes", file=sys.stderr, flush=True)
+    bt = batch.make_batch(specs, device=dev, warmup_iters=400)
+    log = lambda m: print(f"[bench] autotune {m}", file=sys.stderr, flush=True)
+    chosen = bt.autotune(log=log)
+    print(f"[bench] autotune choice {chosen}", file=sys.stderr, flush=True)
+    bt.capture()
+    bt.capture_sequential()
+    for _ in range(args.warmup):
+        bt.run()
+        bt.run_sequential()
+    ms_seq = batch.timed_steps(bt.run_sequential, args.steps, dev)
+    ms = batch.timed_steps(bt.run, args.steps, dev)
+    if not bt.check_capacity():
+        raise RuntimeError("key capacity exceeded during the timed steps")
+    k = {}
+    for name in ("render_fwd", "render_bwd"):
+        k[name] = bt._time(lambda: bt._batched(name), 10)
+    pairs = sum(t.last_pairs for t in bt.trainers) * 256
+    f_fwd, f_bwd = raster_flops_per_pair(3)
+    achieved = pairs * f_bwd / (k["render_bwd"] * 1e-3) / 1e12
+    roof = {"kernel": "k_render_bwd_batch (persistent tile-queue raster adjoint, all scenes)",
+            "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": k["render_bwd"],
+            "pairs": pairs, "flop_per_pair": f_bwd, "fwd_avg_ms": k["render_fwd"]}
+    n = len(bt.trainers)
+    out = {"metric": BATCH_METRIC, "value": n * 1e3 / ms, "unit": "scene-step/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "batch16_uneven_rgb", "scenes": [
+                          [t.W, t.H, t.N] for t in bt.trainers],
+                      "planes": 3, "keys": sum(t.last_pairs for t in bt.trainers),
+                      "sequential_value": n * 1e3 / ms_seq, "sequential_ms_per_round": ms_seq,
+                      "speedup_vs_sequential": ms_seq / ms,
+                      "autotune": {"render_fwd": {"grid": bt.grid["render_fwd"],
+                                                  "chunk": bt.chunk["render_fwd"]},
+                                   "render_bwd": {"grid": bt.grid["render_bwd"],
+                                                  "chunk": bt.chunk["render_bwd"]},
+                                   "streams": bt.n_streams},
+                      "peak_hbm_gb": torch.cuda.max_memory_allocated(dev) / 1e9},
+           "roofline": roof, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -140,6 +197,9 @@ def main():
     import __graft_entry__ as ge
     ge.load_pkg()
     from cvqpg_amd import fused, rasterizer, synthetic, train
+
+    if args.workload == "batch16":
+        return bench_batch(args, torch.device("cuda:0"))
 
     dev = torch.device("cuda:0")
     spec = synthetic.WORKLOADS[args.workload]
