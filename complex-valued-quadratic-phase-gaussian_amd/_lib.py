@@ -67,6 +67,21 @@ SIGNATURES = {
     "cvqpg_metrics_workspace_size": (_c_size, [_c_int] * 4),
     "cvqpg_plane_metrics": (_c_int, [_p, _c_int, _c_int, _p] + [_c_int] * 4 + [_p, _c_size, _p,
                                                                              _p, _p]),
+    "cvqpg_nn_conv": (_c_int, [_p] + [_c_int] * 4 + [_p, _p] + [_c_int] * 3 + [_p] * 5),
+    "cvqpg_nn_conv_wgrad": (_c_int, [_p] + [_c_int] * 4 + [_p, _c_int, _c_int, _p, _p]),
+    "cvqpg_nn_bias_grad": (_c_int, [_p, _c_int, _c_int, _p, _p]),
+    "cvqpg_nn_flip_filter": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p]),
+    "cvqpg_nn_relu_backward": (_c_int, [_p, _p, _c_i64, _p, _p]),
+    "cvqpg_nn_pool2": (_c_int, [_p] + [_c_int] * 4 + [_p, _p]),
+    "cvqpg_nn_pool2_backward": (_c_int, [_p] + [_c_int] * 4 + [_p, _c_int, _p]),
+    "cvqpg_nn_upsample2": (_c_int, [_p] + [_c_int] * 6 + [_p, _p]),
+    "cvqpg_nn_upsample2_backward": (_c_int, [_p] + [_c_int] * 6 + [_p, _c_int, _p]),
+    "cvqpg_nn_tiles_from_image": (_c_int, [_p] + [_c_int] * 4 + [_p, _p]),
+    "cvqpg_nn_wmse_workspace_size": (_c_size, [_c_i64]),
+    "cvqpg_nn_wmse": (_c_int, [_p, _p, _p, _c_i64, _c_f32, _p, _p, _c_size, _p, _p]),
+    "cvqpg_nn_adam": (_c_int, [_p, _p, _p, _p, _p, _c_i64, _c_f32, _c_f32, _c_f32, _c_f32,
+                               _c_int, _p]),
+    "cvqpg_nn_to_bf16": (_c_int, [_p, _c_i64, _p, _p]),
 }
 
 _LIB = None

@@ -303,6 +303,55 @@ int cvqpg_plane_metrics(const float *recon, int recon_height, int recon_width,
                         const float *target, int planes, int channels, int height, int width,
                         void *ws, size_t ws_bytes, float *psnr, float *ssim, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Learned warm start (no counterpart in the reference, whose initialisation */
+/* is random, model_2d_gaussian_rgb.py:40-80): BF16 MFMA training and        */
+/* inference kernels of a small convolutional encoder-decoder.               */
+/* ------------------------------------------------------------------------ */
+/* Activations are NHWC bfloat16 ([batch][height][width][channels], raw uint16 bits);
+ * filters are [cout][ksize*ksize*cin] bfloat16, biases FP32.  Channel counts are
+ * multiples of 16; ksize is 1 or 3 ("same" zero padding, stride 1).
+ * cvqpg_nn_conv: y = act(conv(x, w) + bias); flags bit 0 = ReLU, bit 1 = accumulate
+ * into y_f32.  Outputs (any of): y_bf16; y2_bf16 = bf16(y_bf16 + addend); y_f32. */
+int cvqpg_nn_conv(const uint16_t *x, int batch, int height, int width, int cin,
+                  const uint16_t *w, const float *bias, int cout, int ksize, int flags,
+                  const uint16_t *addend, uint16_t *y_bf16, uint16_t *y2_bf16, float *y_f32,
+                  void *stream);
+/* dw[n][tap*cin + c] = sum over pixels of dy[p][n] * x[p + offset(tap)][c] (FP32 out). */
+int cvqpg_nn_conv_wgrad(const uint16_t *x, int batch, int height, int width, int cin,
+                        const uint16_t *dy, int cout, int ksize, float *dw, void *stream);
+int cvqpg_nn_bias_grad(const uint16_t *dy, int rows, int cout, float *db, void *stream);
+/* Filter of the data gradient: w_flipped[c][tap][n] = w[n][ksize^2 - 1 - tap][c]. */
+int cvqpg_nn_flip_filter(const uint16_t *w, int cout, int cin, int ksize, uint16_t *w_flipped,
+                         void *stream);
+/* dpre = bf16(post > 0 ? dpost : 0) (post NULL: bf16(dpost)). */
+int cvqpg_nn_relu_backward(const float *dpost, const uint16_t *post, int64_t n, uint16_t *dpre,
+                           void *stream);
+/* 2x2 average pooling to ceil(h/2) x ceil(w/2) (mean of the covered pixels) and its
+ * adjoint; nearest-neighbour 2x upsampling from ceil(h/2) x ceil(w/2) to h x w and its
+ * adjoint.  accumulate = 1 adds into dx. */
+int cvqpg_nn_pool2(const uint16_t *x, int batch, int height, int width, int channels,
+                   uint16_t *y, void *stream);
+int cvqpg_nn_pool2_backward(const float *dy, int batch, int height, int width, int channels,
+                            float *dx, int accumulate, void *stream);
+int cvqpg_nn_upsample2(const uint16_t *x, int batch, int src_height, int src_width, int channels,
+                       int height, int width, uint16_t *y, void *stream);
+int cvqpg_nn_upsample2_backward(const float *dy, int batch, int src_height, int src_width,
+                                int channels, int height, int width, float *dx, int accumulate,
+                                void *stream);
+/* img [batch][channels][height][width] FP32 -> one row per 16x16 tile,
+ * tiles [batch][ceil(h/16)][ceil(w/16)][channels*256] bf16, element (c*16 + dy)*16 + dx. */
+int cvqpg_nn_tiles_from_image(const float *img, int batch, int channels, int height, int width,
+                              uint16_t *tiles, void *stream);
+/* loss = sum scale * weight * (out - target)^2 (weight NULL = 1), dout = its gradient. */
+size_t cvqpg_nn_wmse_workspace_size(int64_t n);
+int cvqpg_nn_wmse(const float *out, const float *target, const float *weight, int64_t n,
+                  float scale, float *dout, void *ws, size_t ws_bytes, float *loss, void *stream);
+/* Adam (bias-corrected, step >= 1) on FP32 master weights, refreshing the bf16 copy. */
+int cvqpg_nn_adam(float *w, uint16_t *w_bf16, const float *grad, float *m, float *v, int64_t n,
+                  float lr, float beta1, float beta2, float eps, int step, void *stream);
+int cvqpg_nn_to_bf16(const float *src, int64_t n, uint16_t *dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
