@@ -5,7 +5,9 @@ complex64 field, 3 wavelengths (639/515/473 nm), 8 um pitch, 2x padded BLASM
 (2816 x 4480 FFT grid) to 4 planes at 0, 2, 4, 6 mm; one step = rasterize -> pad ->
 glue -> BLASM -> multi-plane L2 + SSIM loss -> full adjoint -> Adan.
 
-Prints one JSON line (rank 0).  ``roofline`` describes the dominant kernel, timed
+Prints one JSON line (rank 0).  ``time_to_psnr`` is a reduced learned-warm-start study
+(32 training pairs, 2 held-out 1080p scenes; ``--workload warmstart`` runs the full one):
+speed-up of the warm start to the plain fit's final per-plane PSNR.  ``roofline`` describes the dominant kernel, timed
 with HIP events on its launch stream inside the timed region.  ``cpu_baseline``
 times the CPU oracle on a bounded sample of the same workload (see
 ``cpu_baseline``).
@@ -35,9 +37,12 @@ def parse():
                          "path on the GPU (3-sigma / 64-px tile renderer, torch.fft BLASM, "
                          "torch loss and Adan), the denominator of the speed-up target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warmstart", action="store_true",
+                    help="skip the reduced warm-start time-to-PSNR study of the default run")
     ap.add_argument("--workload", default=WORKLOAD,
-                    help="rgb1080p_150k (the metric's workload), gray256_10k, rgb4k_600k, or "
-                         "batch16 (16 uneven scenes, batched vs sequential)")
+                    help="rgb1080p_150k (the metric's workload), gray256_10k, rgb4k_600k, "
+                         "batch16 (16 uneven scenes, batched vs sequential) or warmstart "
+                         "(full learned warm-start time-to-PSNR study)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB after the timed steps (a "
                          "plain copy kernel of known size for scaling HBM byte counters)")
@@ -186,6 +191,38 @@ def bench_batch(args, dev):
     print(json.dumps(out), flush=True)
 
 
+WARM_METRIC = "time-to-PSNR warm-start vs plain (speed-up at equal per-plane PSNR)"
+
+
+def warm_summary(res):
+    """Compact form of warmstart.experiment's result for the JSON line."""
+    out = {"network": res["network"], "evals": []}
+    for e in res["evals"]:
+        d = {k: e[k] for k in ("frame", "gaussians", "plain_iters")}
+        d["net"] = e["net"]
+        d["mean_start"] = e["mean_start"]
+        d["speedup_median"] = (1.0 / e["net"]["median_time_ratio"]
+                               if e["net"]["median_time_ratio"] else None)
+        d["psnr_plain_final"] = [r["plain_final_psnr"] for r in e["rows"]]
+        d["warm_iters"] = [r["warm_iters"] for r in e["rows"]]
+        out["evals"].append(d)
+    return out
+
+
+def bench_warmstart(args, dev, quick=False):
+    """Learned warm start against the plain start: wall-clock (GPU time of the fit steps,
+    plus the network forward and decode for the warm start) to reach the plain fit's
+    final per-plane PSNR within 0.1 dB, on held-out seeded scenes."""
+    from cvqpg_amd import warmstart
+    log = lambda m: print(f"[bench] warmstart {m}", file=sys.stderr, flush=True)
+    if quick:
+        res = warmstart.experiment(dev, n_train=32, fit_iters=600, train_steps=800,
+                                   eval_sets=(((1920, 1080), 2, 1000),), log=log)
+    else:
+        res = warmstart.experiment(dev, log=log)
+    return warm_summary(res)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -200,6 +237,20 @@ def main():
 
     if args.workload == "batch16":
         return bench_batch(args, torch.device("cuda:0"))
+    if args.workload == "warmstart":
+        dev = torch.device("cuda:0")
+        t0 = time.perf_counter()
+        res = bench_warmstart(args, dev)
+        hd = [e for e in res["evals"] if e["frame"] == [1920, 1080]][0]
+        out = {"metric": WARM_METRIC, "value": hd["speedup_median"], "unit": "x",
+               "n_gpus": 1, "steps": None, "warmup": None,
+               "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32 fit, bf16/f32-accumulate network",
+               "data": "synthetic", "config": {"workload": "warmstart", **res,
+                                               "wall_s": time.perf_counter() - t0},
+               "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     spec = synthetic.WORKLOADS[args.workload]
@@ -267,6 +318,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] calibration copy of {args.calib_copy} MiB done", file=sys.stderr, flush=True)
+    warm = None
+    if not args.no_warmstart and args.workload == WORKLOAD:
+        try:
+            warm = bench_warmstart(args, dev, quick=True)
+        except Exception as e:  # the step/s line must still be printed
+            print(f"[bench] warm-start study failed: {e!r}", file=sys.stderr, flush=True)
+            warm = {"error": repr(e)}
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
@@ -280,7 +338,7 @@ def main():
                       "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
                       "peak_hbm_gb": torch.cuda.max_memory_allocated(dev) / 1e9,
                       "stage_ms_eager": stages},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "time_to_psnr": warm}
     print(json.dumps(out), flush=True)
 
 

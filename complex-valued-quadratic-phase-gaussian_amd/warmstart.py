@@ -12,22 +12,25 @@ Network (all on hand-written bf16 MFMA kernels with FP32 accumulation, ``csrc/nn
 one row per 16x16 tile (the tile's 4 x 256 pixels of RGB + depth), a 1x1 stem to 32
 channels, a U-Net over the tile grid with 4 down stages (2x2 average pooling, 3x3
 convolutions, 32-64-128-256-256 channels), 4 up stages (nearest upsampling, 3x3
-convolutions, additive skips) and a 1x1 head to 8 slots x 14 values per tile;
-3.4 M parameters.  Training: weighted squared error against per-tile encodings of the
+convolutions, additive skips) and a 1x1 head to 8 slots x 18 values per tile;
+3.56 M parameters.  Training: weighted squared error against per-tile encodings of the
 Gaussians that plain fits produce, Adam on FP32 master weights with bf16 copies.
 
 Slot encoding (per Gaussian, tile origin (16 tx, 16 ty)):
   0-1   position in the tile, 2 (m - 16 t) / 16 - 1
   2-3   pre-activation log scales (scales = exp(.) + 0.1)
   4-5   cos 2 theta, sin 2 theta (rotation is pi-periodic)
-  6-11  per channel a cos(phi), a sin(phi) (amplitude and phase as one complex number)
-  12    tanh of the curvature parameter (curvature / its scale-aware bound)
-  13    opacity
-Slots hold a tile's Gaussians in decreasing opacity x amplitude^2 x area; empty slots
-encode zero amplitude and zero opacity.
+  6-8   per channel amplitude |a|
+  9-14  per channel cos(phi), sin(phi) of the phase (of a e^{i phi}, so a sign of a is
+        folded into phi)
+  15    tanh of the curvature parameter (curvature / its scale-aware bound)
+  16    opacity
+  17    1 if the slot holds a Gaussian
+Slots hold a tile's Gaussians in decreasing opacity x amplitude^2 x area.  An empty
+slot's target is zero amplitude, opacity and presence, its other values are free (zero
+weight).  Decoded slots with presence below 1/2 keep the random initial Gaussian.
 """
 import math
-import time
 
 import numpy as np
 import torch
@@ -35,7 +38,7 @@ import torch
 from . import _lib
 
 CHANNELS = (32, 64, 128, 256, 256)
-SLOTS, SLOT_DIM = 8, 14
+SLOTS, SLOT_DIM = 8, 18
 OUT_DIM = SLOTS * SLOT_DIM
 IN_CH = 4  # RGB target + depth
 
@@ -195,7 +198,7 @@ class WarmStartNet:
 
     def forward(self, img):
         """img [B, 4, H, W] FP32 (RGB target in [0, 1] and depth in [0, 1]) ->
-        [B, ceil(H/16), ceil(W/16), 112] FP32 slot encodings.  No host synchronisation."""
+        [B, ceil(H/16), ceil(W/16), 144] FP32 slot encodings.  No host synchronisation."""
         img = img.contiguous().float()
         B, C, H, W = img.shape
         assert C == self.in_ch
@@ -299,7 +302,7 @@ class WarmStartNet:
         self._refresh_flips()
 
     def train_step(self, img, target, weight, lr, scale=None):
-        """forward, loss, backward, Adam.  target / weight [B, Ty, Tx, 112].  Returns the
+        """forward, loss, backward, Adam.  target / weight [B, Ty, Tx, 144].  Returns the
         device loss (no host synchronisation)."""
         out = self.forward(img)
         A = self.activations(*out.shape[:3])
@@ -322,7 +325,7 @@ def activated(params, W, H):
 
 
 def encode_targets(params, W, H):
-    """Slot targets [Ty, Tx, 112] and weights of a fitted scene (raw parameters)."""
+    """Slot targets [Ty, Tx, 144] and weights of a fitted scene (raw parameters)."""
     a = activated(params, W, H)
     dev = a["mx"].device
     Ty, Tx = -(-H // 16), -(-W // 16)
@@ -346,18 +349,25 @@ def encode_targets(params, W, H):
     enc[:, 4] = torch.cos(2 * a["rot"][g])
     enc[:, 5] = torch.sin(2 * a["rot"][g])
     for c in range(min(C, 3)):
-        enc[:, 6 + 2 * c] = a["colours"][g, c] * torch.cos(a["phase"][g, c])
-        enc[:, 7 + 2 * c] = a["colours"][g, c] * torch.sin(a["phase"][g, c])
-    enc[:, 12] = a["curv"][g]
-    enc[:, 13] = a["opac"][g]
+        amp, ph = a["colours"][g, c], a["phase"][g, c]
+        ph = torch.where(amp < 0, ph + math.pi, ph)
+        enc[:, 6 + c] = amp.abs()
+        enc[:, 9 + 2 * c] = torch.cos(ph)
+        enc[:, 10 + 2 * c] = torch.sin(ph)
+    enc[:, 15] = a["curv"][g]
+    enc[:, 16] = a["opac"][g]
+    enc[:, 17] = 1.0
     target = torch.zeros(Ty * Tx, SLOTS, SLOT_DIM, device=dev)
     target[tk, rk] = enc
-    weight = torch.ones_like(target)
+    weight = torch.zeros_like(target)
+    weight[:, :, [6, 7, 8, 16, 17]] = 1.0
+    weight[tk, rk] = 1.0
     return target.view(Ty, Tx, OUT_DIM), weight.view(Ty, Tx, OUT_DIM)
 
 
 def decode(out, W, H, channels=3):
-    """Slot encodings [Ty, Tx, 112] -> seven raw parameter tensors of Ty*Tx*8 Gaussians."""
+    """Slot encodings [Ty, Tx, 144] -> (seven raw parameter tensors of Ty*Tx*8 Gaussians,
+    presence mask [Ty*Tx*8])."""
     Ty, Tx = out.shape[:2]
     o = out.reshape(Ty * Tx * SLOTS, SLOT_DIM).float()
     t = torch.arange(Ty * Tx, device=out.device).repeat_interleave(SLOTS)
@@ -369,14 +379,14 @@ def decode(out, W, H, channels=3):
                          torch.atanh((2 * my / H - 1).clamp(-0.9999, 0.9999))], 1)
     scales = o[:, 2:4].clamp(-4.0, 3.0)
     rot = 0.5 * torch.atan2(o[:, 5], o[:, 4])
-    re = torch.stack([o[:, 6 + 2 * c] for c in range(channels)], 1)
-    im = torch.stack([o[:, 7 + 2 * c] for c in range(channels)], 1)
-    colours = torch.sqrt(re * re + im * im)
-    phase = torch.remainder(torch.atan2(im, re), 2 * math.pi)
-    curv = torch.atanh(o[:, 12].clamp(-0.99, 0.99))
-    opac = torch.logit(o[:, 13].clamp(0.01, 0.99))
+    colours = torch.stack([o[:, 6 + c] for c in range(channels)], 1).clamp(min=0.0)
+    phase = torch.stack([torch.atan2(o[:, 10 + 2 * c], o[:, 9 + 2 * c]) for c in range(channels)], 1)
+    phase = torch.remainder(phase, 2 * math.pi)
+    curv = torch.atanh(o[:, 15].clamp(-0.99, 0.99))
+    opac = torch.logit(o[:, 16].clamp(0.01, 0.99))
+    present = o[:, 17] >= 0.5
     return [means.contiguous(), scales.contiguous(), rot.contiguous(), colours.contiguous(),
-            phase.contiguous(), opac.contiguous(), curv.contiguous()]
+            phase.contiguous(), opac.contiguous(), curv.contiguous()], present
 
 
 def network_input(wl):
@@ -384,17 +394,24 @@ def network_input(wl):
     return torch.cat([wl.target_image.float(), wl.target_depth.float()], 0)
 
 
-def warm_params(net, wl):
-    """Raw parameters of a warm-started scene: the network's 8 Gaussians per tile, then the
-    workload's own (random) initial Gaussians up to its count N."""
+def params_from_slots(out, wl):
+    """Raw parameters of a scene started from slot encodings [Ty, Tx, 144]: the decoded
+    Gaussians of the present slots, then the workload's random initial Gaussians up to N."""
     W, H = wl.img_size
-    out = net.forward(network_input(wl)[None])[0]
-    pred = decode(out, W, H, wl.gaussians.channels)
+    pred, present = decode(out, W, H, wl.gaussians.channels)
     from .model import PARAM_NAMES
     N = wl.gaussians.means_2d.shape[0]
     rest = [getattr(wl.gaussians, n).detach() for n in PARAM_NAMES]
-    k = min(N, pred[0].shape[0])
-    return [torch.cat([p[:k], r[:N - k]], 0).contiguous() for p, r in zip(pred, rest)]
+    idx = torch.nonzero(present).flatten()[:N]
+    k = idx.numel()
+    return [torch.cat([p[idx], r[:N - k]], 0).contiguous() for p, r in zip(pred, rest)]
+
+
+def warm_params(net, wl):
+    """Raw parameters of a warm-started scene: the network's Gaussians (up to 8 per tile,
+    the slots it marks present), then the workload's own random initial Gaussians up to
+    its count N."""
+    return params_from_slots(net.forward(network_input(wl)[None])[0], wl)
 
 
 # ---------------------------------------------------------------------- experiment
@@ -497,7 +514,8 @@ def train(net, xs, ts, ws, steps, batch=16, lr=1e-3, seed=0, log=None):
     return out
 
 
-def time_to_psnr(net, spec, seeds, iters, eval_every, device, tol_db=0.1, log=None):
+def time_to_psnr(net, spec, seeds, iters, eval_every, device, tol_db=0.1, log=None,
+                 init_fn=None):
     """Per held-out scene: plain fit for ``iters`` iterations -> final per-plane PSNR P;
     warm-started fit until every plane reaches P - tol_db (at most ``iters`` iterations).
     Times are GPU milliseconds of the fit steps (same step kernel and N for both starts);
@@ -515,7 +533,7 @@ def time_to_psnr(net, spec, seeds, iters, eval_every, device, tol_db=0.1, log=No
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record()
-        init = warm_params(net, wl2)
+        init = warm_params(net, wl2) if init_fn is None else init_fn(wl2)
         b.record()
         warm = fit(wl2, iters, eval_every, num_itrs=iters, init=init, stop_at=need)
         t_net = a.elapsed_time(b)
@@ -544,3 +562,49 @@ def summarize(rows):
                 mean_time_ratio=float(np.mean(ratio)) if ratio else None,
                 median_warm_iters=float(np.median([r["warm_iters"] for r in rows])),
                 plain_iters=rows[0]["plain_iters"] if rows else None)
+
+
+def mean_slots(ts, ws):
+    """Image-independent slot encoding: the weighted mean of the training targets."""
+    return (ts * ws).sum(0) / ws.sum(0).clamp(min=1)
+
+
+def experiment(device="cuda", train_size=(512, 512), n_train=64, fit_iters=600,
+               train_steps=1500, eval_sets=(((512, 512), 8, 1000), ((1920, 1080), 3, 2000)),
+               eval_every=25, seed=0, log=None):
+    """The whole warm-start study on one GPU: training pairs from plain fits, network
+    training, then time-to-PSNR of the warm start against the plain start on held-out
+    scenes (and against an image-independent start from the mean slot encoding, which
+    isolates what the network's conditioning on the image adds).  Returns a dict."""
+    import time as _time
+    dev = torch.device(device)
+    t0 = _time.perf_counter()
+    spec = scene_spec(*train_size)
+    xs, ts, ws = make_pairs(spec, [10_000 + seed * 100_000 + i for i in range(n_train)],
+                            fit_iters, dev, log=log)
+    torch.cuda.synchronize(dev)
+    t_pairs = _time.perf_counter() - t0
+    net = WarmStartNet(dev, seed=seed)
+    t1 = _time.perf_counter()
+    hist = train(net, xs, ts, ws, train_steps, lr=1e-3, seed=seed, log=log)
+    torch.cuda.synchronize(dev)
+    t_train = _time.perf_counter() - t1
+    tm = mean_slots(ts, ws)
+    result = dict(network=dict(params=net.n_params, train_pairs=n_train,
+                               train_frame=list(train_size), fit_iters=fit_iters,
+                               train_steps=train_steps, final_train_loss=hist[-1][1],
+                               pairs_s=t_pairs, train_s=t_train),
+                  evals=[])
+    del xs, ts, ws
+    for (W, H), n, iters in eval_sets:
+        espec = scene_spec(W, H)
+        seeds = [20_000 + seed * 100_000 + i for i in range(n)]
+        rows = time_to_psnr(net, espec, seeds, iters, eval_every, dev, log=log)
+        mean_tm = None
+        if (W, H) == tuple(train_size):
+            rows_mean = time_to_psnr(None, espec, seeds, iters, eval_every, dev, log=log,
+                                     init_fn=lambda wl: params_from_slots(tm, wl))
+            mean_tm = summarize(rows_mean)
+        result["evals"].append(dict(frame=[W, H], gaussians=espec["N"], plain_iters=iters,
+                                    net=summarize(rows), mean_start=mean_tm, rows=rows))
+    return result

@@ -27,7 +27,7 @@ import torch.nn.functional as F
 
 # (name, cin, cout, ksize, relu): the encoder-decoder, level 0 = one row per 16x16 tile
 CHANNELS = (32, 64, 128, 256, 256)
-SLOTS, SLOT_DIM = 8, 14
+SLOTS, SLOT_DIM = 8, 18
 OUT_DIM = SLOTS * SLOT_DIM
 
 
@@ -112,7 +112,7 @@ class _Round(torch.autograd.Function):
 
 
 def forward(img, params, rounding=True):
-    """img [B, C, H, W]; params {name: (w [cout, k*k*cin], b [cout])} -> [B, Ty, Tx, 112]."""
+    """img [B, C, H, W]; params {name: (w [cout, k*k*cin], b [cout])} -> [B, Ty, Tx, 144]."""
     R = _Round.apply if rounding else (lambda t: t)
     spec = {n: (ci, co, k, a) for n, ci, co, k, a in layers(img.shape[1])}
 

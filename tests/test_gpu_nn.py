@@ -207,11 +207,12 @@ def test_encode_decode_round_trip(pkg):
     params = [getattr(wl.gaussians, n).detach() for n in PARAM_NAMES]
     t, w = ws.encode_targets(params, 64, 48)
     assert t.shape == (3, 4, ws.OUT_DIM)
-    dec = ws.decode(t, 64, 48)
+    dec, present = ws.decode(t, 64, 48)
     a = ws.activated(params, 64, 48)
     b = ws.activated(dec, 64, 48)
-    # every decoded Gaussian with nonzero opacity matches one of the originals
-    keep = b["opac"] > 0.02
+    # every decoded Gaussian marked present matches one of the originals
+    keep = present
+    assert bool((b["opac"][~present] < 0.02).all())
     d = (b["mx"][keep, None] - a["mx"][None]) ** 2 + (b["my"][keep, None] - a["my"][None]) ** 2
     j = d.argmin(1)
     assert float(d.min(1).values.max()) < 1e-6

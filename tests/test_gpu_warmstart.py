@@ -1,0 +1,31 @@
+"""What the learned warm start gains over the plain (random) start.
+
+A reduced version of ``warmstart.experiment``: 16 training pairs from 300-iteration
+plain fits of 256x256 RGB scenes (the 1080p recipe's Gaussian density and planes), 300
+network training steps, then two held-out scenes: the plain fit runs 400 iterations and
+its final per-plane PSNR is the bar; the warm-started fit (network Gaussians + random
+ones up to the same N) must reach every plane's bar minus 0.1 dB in at most half the
+time, and no later than the image-independent start from the mean slot encoding.
+"""
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def test_warm_start_reaches_plain_psnr_faster(pkg):
+    from cvqpg_amd import warmstart
+    res = warmstart.experiment(DEV, train_size=(256, 256), n_train=16, fit_iters=300,
+                               train_steps=300, eval_sets=(((256, 256), 2, 400),),
+                               eval_every=25, seed=1,
+                               log=lambda m: print(m, flush=True))
+    e = res["evals"][0]
+    for r in e["rows"]:
+        print(f"seed {r['seed']}: plain {r['plain_iters']} it {r['plain_ms']:.0f} ms "
+              f"final {r['plain_final_psnr']}; warm {r['warm_iters']} it {r['warm_ms']:.0f} ms",
+              flush=True)
+    print("net", e["net"], "mean", e["mean_start"], flush=True)
+    assert e["net"]["warm_reached"] == 2
+    assert e["net"]["median_time_ratio"] <= 0.5
+    assert e["net"]["median_warm_iters"] <= e["mean_start"]["median_warm_iters"]
