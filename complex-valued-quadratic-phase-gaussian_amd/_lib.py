@@ -68,8 +68,9 @@ SIGNATURES = {
     "cvqpg_plane_metrics": (_c_int, [_p, _c_int, _c_int, _p] + [_c_int] * 4 + [_p, _c_size, _p,
                                                                              _p, _p]),
     "cvqpg_nn_conv": (_c_int, [_p] + [_c_int] * 4 + [_p, _p] + [_c_int] * 3 + [_p] * 5),
-    "cvqpg_nn_conv_wgrad": (_c_int, [_p] + [_c_int] * 4 + [_p, _c_int, _c_int, _p, _p]),
-    "cvqpg_nn_bias_grad": (_c_int, [_p, _c_int, _c_int, _p, _p]),
+    "cvqpg_nn_wgrad_workspace_size": (_c_size, [_c_int] * 6),
+    "cvqpg_nn_conv_wgrad": (_c_int, [_p] + [_c_int] * 4 + [_p, _c_int, _c_int, _p, _p, _p,
+                                                          _c_size, _p]),
     "cvqpg_nn_flip_filter": (_c_int, [_p, _c_int, _c_int, _c_int, _p, _p]),
     "cvqpg_nn_relu_backward": (_c_int, [_p, _p, _c_i64, _p, _p]),
     "cvqpg_nn_pool2": (_c_int, [_p] + [_c_int] * 4 + [_p, _p]),

@@ -10,7 +10,8 @@
 // padding) or 1x1, stride 1.  Down- and up-sampling are separate 2x2 average pooling and
 // nearest-neighbour kernels, so the data gradient of every convolution is again a
 // stride-1 convolution (with the flipped, transposed filter) and runs on the same kernel.
-// The weight gradient is a GEMM over the pixels, one wave per 16x16 block of the filter.
+// The weight gradient is a GEMM over the pixels, one wave per 16x16 block of the filter
+// and per 1024-pixel chunk, the chunks added in a fixed order by a second kernel.
 //
 // MFMA operand layout (16x16x16, 64 lanes; r = lane % 16, g = lane / 16):
 //   A (16 x 16, rows x k): lane holds A[r][4g + j], j = 0..3
@@ -106,11 +107,18 @@ __global__ __launch_bounds__(256) void k_conv(const bf16_t* __restrict__ X, int 
   }
 }
 
-// dW[n][tap * Cin + c] = sum_p dY[p][n] * X[p + offset(tap)][c]; one wave per 16x16 block.
+// dW[n][tap * Cin + c] = sum_p dY[p][n] * X[p + offset(tap)][c] and db[n] = sum_p dY[p][n].
+// One wave per 16x16 filter block and per chunk of WG_PIX pixels (gridDim.y chunks); each
+// writes its partial block to P[chunk][n][k] (and, for the k = 0 blocks, four partial
+// bias sums per channel to Q[chunk][g][n]); k_wgrad_reduce adds the chunks in a fixed
+// order, so the result is deterministic.
+constexpr int WG_PIX = 1024;
+
 template <int KS>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t* __restrict__ X, int B, int H,
                                                     int W, int Cin, const bf16_t* __restrict__ dY,
-                                                    int Cout, float* __restrict__ dW) {
+                                                    int Cout, float* __restrict__ P,
+                                                    float* __restrict__ Q) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int K = KS * KS * Cin;
   const int KB = K >> 4;
@@ -121,36 +129,58 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t* __restrict__ X
   const int dy = tap / KS - KS / 2, dx = tap % KS - KS / 2;
   const int r = lane & 15, g = lane >> 4;
   const int M = B * H * W;
+  const int pbeg = blockIdx.y * WG_PIX, pend = min(M, pbeg + WG_PIX);
   f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
-  for (int p0 = 0; p0 < M; p0 += 16) {
+  float bsum = 0.f;
+  for (int p0 = pbeg; p0 < pend; p0 += 16) {
+    // pixel p0 + 4g and its three successors, decoded once and then stepped
+    int p = p0 + 4 * g;
+    int x = p % W, t = p / W;
+    int y = t % H, b = t / H;
     s4v a, w;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int p = p0 + 4 * g + j;
       short av = 0, xv = 0;
-      if (p < M) {
+      if (p < pend) {
         av = (short)dY[(size_t)p * Cout + n0 + r];
-        const int x = p % W, t = p / W;
-        const int y = t % H, b = t / H;
         const int ys = y + dy, xs = x + dx;
         if (ys >= 0 && ys < H && xs >= 0 && xs < W)
           xv = (short)X[((size_t)(b * H + ys) * W + xs) * Cin + c0 + r];
       }
       a[j] = av;
       w[j] = xv;
+      bsum += from_bf16((bf16_t)av);
+      ++p;
+      if (++x == W) {
+        x = 0;
+        if (++y == H) {
+          y = 0;
+          ++b;
+        }
+      }
     }
     acc = mfma16(a, w, acc);
   }
+  float* out = P + (size_t)blockIdx.y * Cout * K;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) dW[(size_t)(n0 + 4 * g + j) * K + k0 + r] = acc[j];
+  for (int j = 0; j < 4; ++j) out[(size_t)(n0 + 4 * g + j) * K + k0 + r] = acc[j];
+  if (k0 == 0) Q[((size_t)blockIdx.y * 4 + g) * Cout + n0 + r] = bsum;
 }
 
-__global__ void k_bias_grad(const bf16_t* __restrict__ dY, int M, int Cout, float* __restrict__ db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= Cout) return;
-  float s = 0.f;
-  for (int p = 0; p < M; ++p) s += from_bf16(dY[(size_t)p * Cout + n]);
-  db[n] = s;
+__global__ void k_wgrad_reduce(const float* __restrict__ P, const float* __restrict__ Q,
+                               int chunks, int64_t nw, int Cout, float* __restrict__ dW,
+                               float* __restrict__ db) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) {
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += P[(size_t)c * nw + i];
+    dW[i] = s;
+  }
+  if (db && i < Cout) {
+    float s = 0.f;
+    for (int c = 0; c < 4 * chunks; ++c) s += Q[(size_t)c * Cout + i];
+    db[i] = s;
+  }
 }
 
 // Wf[c][tap'][n] = Wt[n][KS*KS - 1 - tap'][c]: the filter of the data gradient
@@ -352,28 +382,39 @@ extern "C" int cvqpg_nn_conv(const uint16_t* x, int batch, int height, int width
   return 0;
 }
 
-extern "C" int cvqpg_nn_conv_wgrad(const uint16_t* x, int batch, int height, int width, int cin,
-                                   const uint16_t* dy, int cout, int ksize, float* dw,
-                                   void* stream) {
-  if (int rc = check_conv(batch, height, width, cin, cout, ksize)) return rc;
-  CVQPG_CHECK_ARG(x && dy && dw, "NULL input/gradient/output");
-  const int blocks = (cout / 16) * (ksize * ksize * cin / 16);
-  hipStream_t s = (hipStream_t)stream;
-  if (ksize == 3)
-    hipLaunchKernelGGL(k_conv_wgrad<3>, dim3(ceil_div(blocks, 4)), dim3(256), 0, s, x, batch,
-                       height, width, cin, dy, cout, dw);
-  else
-    hipLaunchKernelGGL(k_conv_wgrad<1>, dim3(ceil_div(blocks, 4)), dim3(256), 0, s, x, batch,
-                       height, width, cin, dy, cout, dw);
-  CVQPG_LAUNCH_CHECK();
-  return 0;
+static int wgrad_chunks(int batch, int height, int width) {
+  return ceil_div((int64_t)batch * height * width, WG_PIX);
 }
 
-extern "C" int cvqpg_nn_bias_grad(const uint16_t* dy, int rows, int cout, float* db,
-                                  void* stream) {
-  CVQPG_CHECK_ARG(dy && db && rows > 0 && cout > 0, "invalid bias-gradient arguments");
-  hipLaunchKernelGGL(k_bias_grad, dim3(ceil_div(cout, 64)), dim3(64), 0, (hipStream_t)stream, dy,
-                     rows, cout, db);
+extern "C" size_t cvqpg_nn_wgrad_workspace_size(int batch, int height, int width, int cin,
+                                               int cout, int ksize) {
+  const size_t chunks = (size_t)wgrad_chunks(batch, height, width);
+  return chunks * ((size_t)cout * ksize * ksize * cin + 4 * (size_t)cout) * sizeof(float) + 256;
+}
+
+extern "C" int cvqpg_nn_conv_wgrad(const uint16_t* x, int batch, int height, int width, int cin,
+                                   const uint16_t* dy, int cout, int ksize, float* dw, float* db,
+                                   void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = check_conv(batch, height, width, cin, cout, ksize)) return rc;
+  CVQPG_CHECK_ARG(x && dy && dw && ws, "NULL input/gradient/output/workspace");
+  CVQPG_CHECK_ARG(ws_bytes >= cvqpg_nn_wgrad_workspace_size(batch, height, width, cin, cout, ksize),
+                  "workspace too small");
+  const int chunks = wgrad_chunks(batch, height, width);
+  CVQPG_CHECK_ARG(chunks <= 65535, "too many pixels for one weight-gradient call");
+  const int64_t nw = (int64_t)cout * ksize * ksize * cin;
+  float* P = static_cast<float*>(ws);
+  float* Q = P + (size_t)chunks * nw;
+  const int blocks = (cout / 16) * (ksize * ksize * cin / 16);
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid(ceil_div(blocks, 4), chunks);
+  if (ksize == 3)
+    hipLaunchKernelGGL(k_conv_wgrad<3>, grid, dim3(256), 0, s, x, batch, height, width, cin, dy,
+                       cout, P, Q);
+  else
+    hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(256), 0, s, x, batch, height, width, cin, dy,
+                       cout, P, Q);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(nw > cout ? nw : cout)), dim3(256), 0, s, P,
+                     Q, chunks, nw, cout, dw, db);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }

@@ -20,6 +20,7 @@ Target: 40 random anisotropic Gaussian blobs with per-channel amplitudes plus 0.
   to [0, 1].
 Depth: uniform noise smoothed with sigma = 16 px, rescaled to [0, 1].
 """
+import functools
 import math
 from argparse import Namespace
 
@@ -140,6 +141,13 @@ def make_target(C, W, H, seed=0, blobs=40):
     img -= img.min()
     img /= max(img.max(), 1e-12)
     return img.astype(np.float32)
+
+
+@functools.lru_cache(maxsize=2)
+def frame_targets(C, W, H, seed=0):
+    """(make_target, make_depth) of one frame, cached so that several windows of the same
+    frame are cut from one synthesis."""
+    return make_target(C, W, H, seed), make_depth(W, H, seed)
 
 
 def make_depth(W, H, seed=0):

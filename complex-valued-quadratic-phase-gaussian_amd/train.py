@@ -81,8 +81,14 @@ class Workload:
         with torch.no_grad():
             for n in PARAM_NAMES:
                 getattr(self.gaussians, n).copy_(torch.as_tensor(raw[n]))
-        self.target_image = torch.as_tensor(synthetic.make_target(C, W, H, seed)).to(self.device)
-        self.target_depth = torch.as_tensor(synthetic.make_depth(W, H, seed)).to(self.device)
+        if "crop" in spec:  # a W x H window of a larger synthetic frame
+            x0, y0, Ws, Hs = spec["crop"]
+            img, dep = synthetic.frame_targets(C, Ws, Hs, seed)
+            img, dep = img[:, y0:y0 + H, x0:x0 + W], dep[:, y0:y0 + H, x0:x0 + W]
+        else:
+            img, dep = synthetic.make_target(C, W, H, seed), synthetic.make_depth(W, H, seed)
+        self.target_image = torch.as_tensor(img.copy()).to(self.device)
+        self.target_depth = torch.as_tensor(dep.copy()).to(self.device)
         targets, self.loss_fn, self.masks = loss_mod.multiplane_loss(
             self.target_image, self.target_depth, self.args_prop)
         self.targets = targets

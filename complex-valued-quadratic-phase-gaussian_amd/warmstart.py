@@ -14,7 +14,13 @@ channels, a U-Net over the tile grid with 4 down stages (2x2 average pooling, 3x
 convolutions, 32-64-128-256-256 channels), 4 up stages (nearest upsampling, 3x3
 convolutions, additive skips) and a 1x1 head to 8 slots x 18 values per tile;
 3.56 M parameters.  Training: weighted squared error against per-tile encodings of the
-Gaussians that plain fits produce, Adam on FP32 master weights with bf16 copies.
+Gaussians that plain fits produce, Adam on FP32 master weights with bf16 copies.  The
+head's bias starts at the mean slot encoding of the training set, so the untrained
+network is the image-independent "mean start" and training learns deviations from it.
+Decoding takes a predicted phase only where its (cos, sin) vector is long enough to be
+a confident prediction (|v| >= 1/2) and the mean slot's phase elsewhere: phases of
+fitted Gaussians are largely unpredictable from the image, and a start whose uncertain
+phases are coherent converges much faster than one whose phases scatter.
 
 Slot encoding (per Gaussian, tile origin (16 tx, 16 ty)):
   0-1   position in the tile, 2 (m - 16 t) / 16 - 1
@@ -127,6 +133,14 @@ class WarmStartNet:
         a, b = self.offsets[name]
         return self.wb[a:b]
 
+    def set_output_bias(self, vec):
+        """Head bias = ``vec`` [144]: with the head's small initial filter the untrained
+        network then predicts about ``vec`` everywhere and learns image-dependent
+        deviations from it."""
+        _, bv = self.view(self.w32, "head")
+        bv.copy_(vec.to(bv))
+        self._sync_bf16()
+
     def state_dict(self):
         return {n: tuple(t.detach().cpu().clone() for t in self.view(self.w32, n))
                 for n, *_ in self.layers}
@@ -180,6 +194,15 @@ class WarmStartNet:
         big = max(B * h * w * ch for (h, w), ch in zip(lv, c))
         big = max(big, B * lv[0][0] * lv[0][1] * OUT_DIM)
         A["dpre"] = torch.empty(big, dtype=torch.bfloat16, device=dev)
+        lvl = {"stem": 0, "e0": 0, "head": 0, "e4": 4}
+        for l in (1, 2, 3):
+            lvl[f"e{l}a"] = lvl[f"e{l}b"] = l
+        for l in (0, 1, 2, 3):
+            lvl[f"d{l}a"] = lvl[f"d{l}b"] = l
+        L = _lib.lib()
+        nws = max(int(L.cvqpg_nn_wgrad_workspace_size(B, *lv[lvl[n]], ci, co, k))
+                  for n, ci, co, k, _ in self.layers)
+        A["wgrad_ws"] = torch.empty(nws, dtype=torch.uint8, device=dev)
         n_out = B * lv[0][0] * lv[0][1] * OUT_DIM
         A["loss_ws"] = torch.empty(int(_lib.lib().cvqpg_nn_wmse_workspace_size(n_out)),
                                    dtype=torch.uint8, device=dev)
@@ -245,8 +268,8 @@ class WarmStartNet:
                                             _lib.ptr(dpre), s))
         gw, gb = self.view(self.grad, name)
         _lib.check(L.cvqpg_nn_conv_wgrad(_lib.ptr(x), B, h, w, ci, _lib.ptr(dpre), co, k,
-                                         _lib.ptr(gw), s))
-        _lib.check(L.cvqpg_nn_bias_grad(_lib.ptr(dpre), B * h * w, co, _lib.ptr(gb), s))
+                                         _lib.ptr(gw), _lib.ptr(gb), _lib.ptr(A["wgrad_ws"]),
+                                         A["wgrad_ws"].numel(), s))
         if dx is not None:
             _lib.check(L.cvqpg_nn_conv(_lib.ptr(dpre), B, h, w, co, _lib.ptr(self.flip[name]), None,
                                        ci, k, 2 if accumulate else 0, None, None, None,
@@ -365,9 +388,11 @@ def encode_targets(params, W, H):
     return target.view(Ty, Tx, OUT_DIM), weight.view(Ty, Tx, OUT_DIM)
 
 
-def decode(out, W, H, channels=3):
+def decode(out, W, H, channels=3, ref=None, min_norm=0.5):
     """Slot encodings [Ty, Tx, 144] -> (seven raw parameter tensors of Ty*Tx*8 Gaussians,
-    presence mask [Ty*Tx*8])."""
+    presence mask [Ty*Tx*8]).  With ``ref`` (a [144] encoding), a slot's phase whose
+    predicted (cos, sin) vector is shorter than ``min_norm`` (an uncertain phase) takes
+    the reference slot's phase instead."""
     Ty, Tx = out.shape[:2]
     o = out.reshape(Ty * Tx * SLOTS, SLOT_DIM).float()
     t = torch.arange(Ty * Tx, device=out.device).repeat_interleave(SLOTS)
@@ -381,6 +406,12 @@ def decode(out, W, H, channels=3):
     rot = 0.5 * torch.atan2(o[:, 5], o[:, 4])
     colours = torch.stack([o[:, 6 + c] for c in range(channels)], 1).clamp(min=0.0)
     phase = torch.stack([torch.atan2(o[:, 10 + 2 * c], o[:, 9 + 2 * c]) for c in range(channels)], 1)
+    if ref is not None:
+        r = ref.reshape(SLOTS, SLOT_DIM).float().repeat(Ty * Tx, 1)
+        for c in range(channels):
+            norm = torch.hypot(o[:, 9 + 2 * c], o[:, 10 + 2 * c])
+            rp = torch.atan2(r[:, 10 + 2 * c], r[:, 9 + 2 * c])
+            phase[:, c] = torch.where(norm < min_norm, rp, phase[:, c])
     phase = torch.remainder(phase, 2 * math.pi)
     curv = torch.atanh(o[:, 15].clamp(-0.99, 0.99))
     opac = torch.logit(o[:, 16].clamp(0.01, 0.99))
@@ -394,11 +425,11 @@ def network_input(wl):
     return torch.cat([wl.target_image.float(), wl.target_depth.float()], 0)
 
 
-def params_from_slots(out, wl):
+def params_from_slots(out, wl, ref=None):
     """Raw parameters of a scene started from slot encodings [Ty, Tx, 144]: the decoded
     Gaussians of the present slots, then the workload's random initial Gaussians up to N."""
     W, H = wl.img_size
-    pred, present = decode(out, W, H, wl.gaussians.channels)
+    pred, present = decode(out, W, H, wl.gaussians.channels, ref=ref)
     from .model import PARAM_NAMES
     N = wl.gaussians.means_2d.shape[0]
     rest = [getattr(wl.gaussians, n).detach() for n in PARAM_NAMES]
@@ -407,11 +438,11 @@ def params_from_slots(out, wl):
     return [torch.cat([p[idx], r[:N - k]], 0).contiguous() for p, r in zip(pred, rest)]
 
 
-def warm_params(net, wl):
+def warm_params(net, wl, ref=None):
     """Raw parameters of a warm-started scene: the network's Gaussians (up to 8 per tile,
     the slots it marks present), then the workload's own random initial Gaussians up to
     its count N."""
-    return params_from_slots(net.forward(network_input(wl)[None])[0], wl)
+    return params_from_slots(net.forward(network_input(wl)[None])[0], wl, ref=ref)
 
 
 # ---------------------------------------------------------------------- experiment
@@ -469,13 +500,32 @@ def scene_spec(W, H, density=150_000 / (1920 * 1080), num_planes=4, d_val=3e-3,
                 kappa=0.02)
 
 
+def crop_specs(size, source, n, seed=0):
+    """Specs of n training windows of ``size`` cut at seeded offsets from frames of the
+    ``source`` size (4 windows per frame), so that training sees the target and depth
+    statistics of the large frame at the cost of fitting the small one.  Returns
+    (specs, seeds): window i uses frame seed seeds[i]."""
+    W, H = size
+    Ws, Hs = source
+    rng = np.random.default_rng(seed + 31)
+    specs, seeds = [], []
+    for i in range(n):
+        sp = scene_spec(W, H)
+        sp["crop"] = (int(rng.integers(0, Ws - W + 1)), int(rng.integers(0, Hs - H + 1)), Ws, Hs)
+        specs.append(sp)
+        seeds.append(40_000 + seed * 100_000 + i // 4)
+    return specs, seeds
+
+
 def make_pairs(spec, seeds, iters, device, batch=16, log=None):
-    """Training pairs from plain fits: inputs [n, 4, H, W], slot targets and weights."""
+    """Training pairs from plain fits: inputs [n, 4, H, W], slot targets and weights.
+    ``spec`` is one spec for every seed or a list of specs (one per seed)."""
     from .batch import BatchTrainer
     from .fused import FusedTrainer
+    specs = spec if isinstance(spec, (list, tuple)) else [spec] * len(seeds)
     xs, ts, ws = [], [], []
     for i in range(0, len(seeds), batch):
-        wls = [make_workload(spec, s, device) for s in seeds[i:i + batch]]
+        wls = [make_workload(sp, s, device) for sp, s in zip(specs[i:i + batch], seeds[i:i + batch])]
         trs = [FusedTrainer(wl, num_itrs=iters, warmup_iters=min(400, iters // 5)) for wl in wls]
         bt = BatchTrainer(None, trainers=trs, streams=4)
         bt.capture()
@@ -484,7 +534,7 @@ def make_pairs(spec, seeds, iters, device, batch=16, log=None):
             if (it + 1) % 200 == 0 and not bt.check_capacity():
                 bt.grow()
                 bt.capture()
-        W, H = spec["W"], spec["H"]
+        W, H = specs[i]["W"], specs[i]["H"]
         for wl, tr in zip(wls, trs):
             t, w = encode_targets(tr.params, W, H)
             xs.append(network_input(wl))
@@ -508,19 +558,22 @@ def train(net, xs, ts, ws, steps, batch=16, lr=1e-3, seed=0, log=None):
         loss = net.train_step(xs[idx], ts[idx], ws[idx], lr_k)
         if k % 50 == 0 or k == steps - 1:
             hist.append((k, loss.clone()))
+        if log and k % 250 == 249:
+            log(f"train step {k + 1}/{steps}: loss {float(loss):.4f}")
     out = [(k, float(v)) for k, v in hist]
     if log:
         log("train loss " + " ".join(f"{k}:{v:.4f}" for k, v in out[::max(1, len(out) // 8)]))
     return out
 
 
-def time_to_psnr(net, spec, seeds, iters, eval_every, device, tol_db=0.1, log=None,
-                 init_fn=None):
+def time_to_psnr(spec, seeds, iters, eval_every, device, starts, tol_db=0.1, log=None):
     """Per held-out scene: plain fit for ``iters`` iterations -> final per-plane PSNR P;
-    warm-started fit until every plane reaches P - tol_db (at most ``iters`` iterations).
-    Times are GPU milliseconds of the fit steps (same step kernel and N for both starts);
-    the warm start adds the network forward and decode, timed separately."""
-    rows = []
+    then, for each named start in ``starts`` (name -> function(workload) -> raw
+    parameters), a fit from that start until every plane reaches P - tol_db (at most
+    ``iters`` iterations).  Times are GPU milliseconds of the fit steps (same step kernel
+    and N for every start); a start's own cost (network forward, decode) is added to it.
+    Returns {name: [row per scene]}."""
+    out = {name: [] for name in starts}
     for s in seeds:
         wl = make_workload(spec, s, device)
         plain = fit(wl, iters, eval_every)
@@ -528,30 +581,33 @@ def time_to_psnr(net, spec, seeds, iters, eval_every, device, tol_db=0.1, log=No
         need = (final - tol_db).tolist()
         first_hit = next((i for i, p in enumerate(plain["psnr"]) if bool((p >= final - tol_db).all())),
                          len(plain["psnr"]) - 1)
-        wl2 = make_workload(spec, s, device)
-        torch.cuda.synchronize()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        init = warm_params(net, wl2) if init_fn is None else init_fn(wl2)
-        b.record()
-        warm = fit(wl2, iters, eval_every, num_itrs=iters, init=init, stop_at=need)
-        t_net = a.elapsed_time(b)
-        row = dict(seed=s, plain_final_psnr=[round(float(v), 3) for v in final],
-                   plain_iters=plain["iters"], plain_ms=plain["ms"],
-                   plain_iters_to_target=1 + (first_hit + 1) * eval_every,
-                   warm_iters=warm["iters"], warm_ms=warm["ms"] + t_net, warm_net_ms=t_net,
-                   warm_reached=warm["reached"],
-                   psnr_first_eval_plain=[round(float(v), 3) for v in plain["psnr"][0]],
-                   psnr_first_eval_warm=[round(float(v), 3) for v in warm["psnr"][0]])
-        rows.append(row)
-        if log:
-            log(f"seed {s}: plain {plain['iters']} it {plain['ms']:.0f} ms -> "
-                f"{row['plain_final_psnr']}; warm {warm['iters']} it {row['warm_ms']:.0f} ms "
-                f"reached={warm['reached']}; first-eval PSNR plain {row['psnr_first_eval_plain']} "
-                f"warm {row['psnr_first_eval_warm']}")
-        del plain, warm
-    return rows
+        del wl
+        for name, init_fn in starts.items():
+            wl2 = make_workload(spec, s, device)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            init = init_fn(wl2)
+            b.record()
+            warm = fit(wl2, iters, eval_every, num_itrs=iters, init=init, stop_at=need)
+            t_net = a.elapsed_time(b)
+            row = dict(seed=s, plain_final_psnr=[round(float(v), 3) for v in final],
+                       plain_iters=plain["iters"], plain_ms=plain["ms"],
+                       plain_iters_to_target=1 + (first_hit + 1) * eval_every,
+                       warm_iters=warm["iters"], warm_ms=warm["ms"] + t_net, warm_net_ms=t_net,
+                       warm_reached=warm["reached"],
+                       psnr_first_eval_plain=[round(float(v), 3) for v in plain["psnr"][0]],
+                       psnr_first_eval_warm=[round(float(v), 3) for v in warm["psnr"][0]])
+            out[name].append(row)
+            if log:
+                log(f"seed {s} [{name}]: plain {plain['iters']} it {plain['ms']:.0f} ms -> "
+                    f"{row['plain_final_psnr']}; start {warm['iters']} it {row['warm_ms']:.0f} ms "
+                    f"reached={warm['reached']}; first-eval PSNR plain "
+                    f"{row['psnr_first_eval_plain']} start {row['psnr_first_eval_warm']}")
+            del warm, wl2
+        del plain
+    return out
 
 
 def summarize(rows):
@@ -565,46 +621,57 @@ def summarize(rows):
 
 
 def mean_slots(ts, ws):
-    """Image-independent slot encoding: the weighted mean of the training targets."""
-    return (ts * ws).sum(0) / ws.sum(0).clamp(min=1)
+    """Image-independent slot encoding [144]: the weighted mean of the training targets
+    over pairs and tiles (broadcast to any tile grid)."""
+    t = ts.reshape(-1, OUT_DIM)
+    w = ws.reshape(-1, OUT_DIM)
+    return (t * w).sum(0) / w.sum(0).clamp(min=1)
 
 
-def experiment(device="cuda", train_size=(512, 512), n_train=64, fit_iters=600,
-               train_steps=1500, eval_sets=(((512, 512), 8, 1000), ((1920, 1080), 3, 2000)),
-               eval_every=25, seed=0, log=None):
-    """The whole warm-start study on one GPU: training pairs from plain fits, network
-    training, then time-to-PSNR of the warm start against the plain start on held-out
-    scenes (and against an image-independent start from the mean slot encoding, which
-    isolates what the network's conditioning on the image adds).  Returns a dict."""
+def experiment(device="cuda", train_size=(1920, 1080), n_train=32, fit_iters=600,
+               train_steps=1500, eval_sets=(((1920, 1080), 4, 2000), ((512, 512), 4, 1000)),
+               eval_every=25, seed=0, log=None, train_source=None, with_mean=True):
+    """The whole warm-start study on one GPU: training pairs from plain fits (windows of
+    ``train_size`` cut from ``train_source`` frames, or whole ``train_size`` frames when it
+    is None), network training, then time-to-PSNR of the warm start against the plain
+    start on held-out scenes, and of an image-independent start from the mean slot
+    encoding, which isolates what the network's conditioning on the image adds.  Returns
+    a dict."""
     import time as _time
     dev = torch.device(device)
     t0 = _time.perf_counter()
-    spec = scene_spec(*train_size)
-    xs, ts, ws = make_pairs(spec, [10_000 + seed * 100_000 + i for i in range(n_train)],
-                            fit_iters, dev, log=log)
+    if train_source is None:
+        spec = scene_spec(*train_size)
+        seeds = [10_000 + seed * 100_000 + i for i in range(n_train)]
+    else:
+        spec, seeds = crop_specs(train_size, train_source, n_train, seed)
+    xs, ts, ws = make_pairs(spec, seeds, fit_iters, dev, log=log)
     torch.cuda.synchronize(dev)
     t_pairs = _time.perf_counter() - t0
+    tm = mean_slots(ts, ws)
     net = WarmStartNet(dev, seed=seed)
+    net.set_output_bias(tm)   # the network starts as the image-blind mean predictor
     t1 = _time.perf_counter()
     hist = train(net, xs, ts, ws, train_steps, lr=1e-3, seed=seed, log=log)
     torch.cuda.synchronize(dev)
     t_train = _time.perf_counter() - t1
-    tm = mean_slots(ts, ws)
     result = dict(network=dict(params=net.n_params, train_pairs=n_train,
-                               train_frame=list(train_size), fit_iters=fit_iters,
+                               train_frame=list(train_size),
+                               train_source=list(train_source) if train_source else None,
+                               fit_iters=fit_iters,
                                train_steps=train_steps, final_train_loss=hist[-1][1],
                                pairs_s=t_pairs, train_s=t_train),
                   evals=[])
     del xs, ts, ws
+    grid = lambda wl: tm.expand(-(-wl.img_size[1] // 16), -(-wl.img_size[0] // 16), OUT_DIM)
+    starts = {"net": lambda wl: warm_params(net, wl, ref=tm)}
+    if with_mean:
+        starts["mean"] = lambda wl: params_from_slots(grid(wl), wl)
     for (W, H), n, iters in eval_sets:
         espec = scene_spec(W, H)
         seeds = [20_000 + seed * 100_000 + i for i in range(n)]
-        rows = time_to_psnr(net, espec, seeds, iters, eval_every, dev, log=log)
-        mean_tm = None
-        if (W, H) == tuple(train_size):
-            rows_mean = time_to_psnr(None, espec, seeds, iters, eval_every, dev, log=log,
-                                     init_fn=lambda wl: params_from_slots(tm, wl))
-            mean_tm = summarize(rows_mean)
-        result["evals"].append(dict(frame=[W, H], gaussians=espec["N"], plain_iters=iters,
-                                    net=summarize(rows), mean_start=mean_tm, rows=rows))
+        rows = time_to_psnr(espec, seeds, iters, eval_every, dev, starts, log=log)
+        result["evals"].append(dict(
+            frame=[W, H], gaussians=espec["N"], plain_iters=iters, net=summarize(rows["net"]),
+            mean_start=summarize(rows["mean"]) if with_mean else None, rows=rows["net"]))
     return result

@@ -317,10 +317,14 @@ int cvqpg_nn_conv(const uint16_t *x, int batch, int height, int width, int cin,
                   const uint16_t *w, const float *bias, int cout, int ksize, int flags,
                   const uint16_t *addend, uint16_t *y_bf16, uint16_t *y2_bf16, float *y_f32,
                   void *stream);
-/* dw[n][tap*cin + c] = sum over pixels of dy[p][n] * x[p + offset(tap)][c] (FP32 out). */
+/* dw[n][tap*cin + c] = sum over pixels of dy[p][n] * x[p + offset(tap)][c] and
+ * db[n] = sum over pixels of dy[p][n] (db may be NULL); FP32, deterministic (per-chunk
+ * partials in the workspace, added in a fixed order). */
+size_t cvqpg_nn_wgrad_workspace_size(int batch, int height, int width, int cin, int cout,
+                                     int ksize);
 int cvqpg_nn_conv_wgrad(const uint16_t *x, int batch, int height, int width, int cin,
-                        const uint16_t *dy, int cout, int ksize, float *dw, void *stream);
-int cvqpg_nn_bias_grad(const uint16_t *dy, int rows, int cout, float *db, void *stream);
+                        const uint16_t *dy, int cout, int ksize, float *dw, float *db, void *ws,
+                        size_t ws_bytes, void *stream);
 /* Filter of the data gradient: w_flipped[c][tap][n] = w[n][ksize^2 - 1 - tap][c]. */
 int cvqpg_nn_flip_filter(const uint16_t *w, int cout, int cin, int ksize, uint16_t *w_flipped,
                          void *stream);

@@ -34,7 +34,8 @@ def rnd_bf16(shape, seed, scale=1.0):
 
 
 @pytest.mark.parametrize("k,B,H,W,cin,cout", [(3, 2, 5, 7, 32, 48), (1, 1, 3, 4, 1024, 32),
-                                              (3, 3, 9, 6, 64, 112), (3, 1, 1, 1, 256, 256)])
+                                              (3, 3, 9, 6, 64, 112), (3, 1, 1, 1, 256, 256),
+                                              (3, 2, 37, 45, 32, 32)])  # 3 pixel chunks
 def test_conv_forward_wgrad_dgrad(pkg, k, B, H, W, cin, cout):
     _lib = lib()
     L = _lib.lib()
@@ -57,8 +58,11 @@ def test_conv_forward_wgrad_dgrad(pkg, k, B, H, W, cin, cout):
     dy = rnd_bf16((B, H, W, cout), 4)
     dyd = dy.to(DEV)
     dw = torch.empty(cout, k * k * cin, device=DEV)
+    db = torch.empty(cout, device=DEV)
+    wsz = int(L.cvqpg_nn_wgrad_workspace_size(B, H, W, cin, cout, k))
+    wws = torch.empty(wsz, dtype=torch.uint8, device=DEV)
     _lib.check(L.cvqpg_nn_conv_wgrad(_lib.ptr(xd), B, H, W, cin, _lib.ptr(dyd), cout, k,
-                                     _lib.ptr(dw), s))
+                                     _lib.ptr(dw), _lib.ptr(db), _lib.ptr(wws), wsz, s))
     x64 = x.double().requires_grad_()
     w64 = w.double().requires_grad_()
     (onn.conv(x64, w64, None, k) * dy.double()).sum().backward()
@@ -69,8 +73,6 @@ def test_conv_forward_wgrad_dgrad(pkg, k, B, H, W, cin, cout):
     _lib.check(L.cvqpg_nn_conv(_lib.ptr(dyd), B, H, W, cout, _lib.ptr(wf), None, cin, k, 0, None,
                                None, None, _lib.ptr(dx), s))
     ex = rel(dx, x64.grad)
-    db = torch.empty(cout, device=DEV)
-    _lib.check(L.cvqpg_nn_bias_grad(_lib.ptr(dyd), B * H * W, cout, _lib.ptr(db), s))
     eb = rel(db, dy.double().sum((0, 1, 2)))
     print(f"  wgrad rel {ew:.1e}, dgrad rel {ex:.1e}, bias rel {eb:.1e}", flush=True)
     assert ew <= 1e-5 and ex <= 1e-5 and eb <= 1e-5
