@@ -527,17 +527,24 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       u[1] = q2.z; w[1] = q2.w; col[1] = q3.x;
       u[C - 1] = q3.y; w[C - 1] = q3.z; col[C - 1] = q3.w;
     }
+    float colh[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) kA[c] = kc[c] * col[c];
+    for (int c = 0; c < C; ++c) {
+      kA[c] = kc[c] * col[c];
+      colh[c] = -0.5f * col[c];
+    }
     // Per pixel the quadratic form is q = dx (a00 dx + 2 a01 dy) + a11 dy^2 (two FMAs
     // from per-row constants), and the pixel sums that need dx or dy are kept as the
     // row moments S0 = sum h, S1 = sum h dx, S2 = sum h dx^2 of h = dL/dq / o, folded
     // into the mean and inverse-covariance gradients once per row.  The opacity factor
-    // of alpha = o G is applied once at the end.
+    // of alpha = o G is applied once at the end, and dL/do = sum_c col_c dL/dcol_c / o
+    // is formed from the colour sums after the loop.  Pairs beyond the power clamp
+    // (q < log2(e^-50)) get G = 2^q instead of e^-50 and their exponent-path term is
+    // not masked: both change the pair's terms by less than e^-50 ~ 2e-22.
     float dcol[C], dph[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) dcol[c] = dph[c] = 0.f;
-    float dop = 0.f, K = 0.f, di00 = 0.f, di01 = 0.f, di11 = 0.f, dmx = 0.f, dmy = 0.f;
+    float K = 0.f, di00 = 0.f, di01 = 0.f, di11 = 0.f, dmx = 0.f, dmy = 0.f;
     const float dx0 = mx - (float)x0;
     for (int yy = 0; yy < ny; ++yy) {
       const float dy = my - (float)(y0 + yy);
@@ -547,8 +554,8 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       auto pixel = [&](int xx) {
         const float dx = dx0 - (float)xx;
         const float q = fmaf(dx, fmaf(a00, dx, bq), cq);
-        const float G = fast_exp2(fmaxf(q, QMIN));
-        float Sa = 0.f, Tb = 0.f;
+        const float G = fast_exp2(q);
+        float Sh = 0.f, Tb = 0.f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float2 gg = gr0[c * 256 + xx];
@@ -558,12 +565,10 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
           const float Qc = fmaf(gg.y, cs, -gg.x * sn);
           dcol[c] = fmaf(G, Pc, dcol[c]);
           dph[c] = fmaf(G, Qc, dph[c]);
-          Sa = fmaf(col[c], Pc, Sa);
+          Sh = fmaf(colh[c], Pc, Sh);
           Tb = fmaf(kA[c], Qc, Tb);
         }
-        dop = fmaf(Sa, G, dop);
-        const float sa = (q >= QMIN) ? Sa : 0.f;
-        const float h = G * fmaf(Tb, beta, -0.5f * sa);
+        const float h = G * fmaf(Tb, beta, Sh);
         K = fmaf(G * Tb, q, K);
         S0 += h;
         const float hd = h * dx;
@@ -582,8 +587,10 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       dmx += fmaf(a00, S1, a01 * dy * S0);
       dmy += fmaf(a01, S1, a11 * dy * S0);
     }
+    float dop = 0.f;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
+      dop = fmaf(col[c], dcol[c], dop);
       dcol[c] *= o;
       dph[c] *= o;
     }
