@@ -437,10 +437,11 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
       const float ex = fmaf(p0.w, dy, p0.z * dx);
       const float ey = fmaf(p1.x, dy, p0.w * dx);
       const float q = fmaf(dy, ey, dx * ex);
-      // pairs beyond the power clamp (power < -50) add o e^-50 ~ 2e-22 per channel;
-      // a wave whose 64 pixels are all there skips the Gaussian
+      // pairs beyond the power clamp (power < -50) add at most o e^-50 ~ 2e-22 per
+      // channel: a wave whose 64 pixels are all there skips the Gaussian, and in other
+      // waves such pixels get 2^q < e^-50 instead of the clamped value
       if (__ballot(q >= QMIN) == 0ull) continue;
-      const float G = fast_exp2(fmaxf(q, QMIN));
+      const float G = fast_exp2(q);
       {
         const float rev = fract_f(fmaf(p1.z, q, p1.y));
         const float a = p1.w * G;
