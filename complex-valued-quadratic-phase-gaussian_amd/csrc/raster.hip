@@ -40,10 +40,13 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_ROUNDS = 16;
 constexpr int SORT_BLOCK = SORT_THREADS * SORT_ROUNDS;  // 4096 keys per block
 constexpr int REC = 16;                                 // floats per packed Gaussian / partial record
-// clamp(-mahal/2, min=-50) expressed on the base-2 exponent q = -mahal/2 * log2(e)
+// G = exp(-mahal/2) is evaluated as 2^q with q = -mahal/2 * log2(e).  The reference clamps
+// the exponent at -50 (q = -72.1); pairs there contribute below e^-50 ~ 2e-22 and the
+// kernels evaluate them as 2^q unclamped (see the comments at the uses).
 constexpr float LOG2E = 1.4426950408889634f;
 constexpr float QSCALE = -0.5f * LOG2E;
-constexpr float QMIN = -50.0f * LOG2E;
+// forward wave skip: all pixels of the wave below G = 2^-24
+constexpr float QSKIP = -24.0f;
 constexpr float INV_2PI = 0.15915494309189535f;
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -437,10 +440,11 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
       const float ex = fmaf(p0.w, dy, p0.z * dx);
       const float ey = fmaf(p1.x, dy, p0.w * dx);
       const float q = fmaf(dy, ey, dx * ex);
-      // pairs beyond the power clamp (power < -50) add at most o e^-50 ~ 2e-22 per
-      // channel: a wave whose 64 pixels are all there skips the Gaussian, and in other
-      // waves such pixels get 2^q < e^-50 instead of the clamped value
-      if (__ballot(q >= QMIN) == 0ull) continue;
+      // a wave whose 64 pixels all have G = 2^q < 2^-24 skips the Gaussian: each such
+      // pair would add less than 2^-24 a o, below FP32 resolution of a unit-amplitude
+      // term (the reference evaluates it, down to its clamp at e^-50); in other waves
+      // pixels beyond the clamp get 2^q < e^-50 instead of the clamped value
+      if (__ballot(q >= QSKIP) == 0ull) continue;
       const float G = fast_exp2(q);
       {
         const float rev = fract_f(fmaf(p1.z, q, p1.y));
