@@ -73,11 +73,15 @@ __device__ __forceinline__ void normals4(unsigned long long seed, uint32_t step,
   const float u2 = (float)(c1 >> 8) * (1.0f / 16777216.0f);
   const float u3 = ((float)(c2 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float u4 = (float)(c3 >> 8) * (1.0f / 16777216.0f);
-  const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
-  float s1, c1f, s2, c2f;
-  sincospif(2.0f * u2, &s1, &c1f);
-  sincospif(2.0f * u4, &s2, &c2f);
-  z[0] = r1 * c1f; z[1] = r1 * s1; z[2] = r2 * c2f; z[3] = r2 * s2;
+  // Box-Muller on the hardware units: -2 ln u = -2 ln2 log2 u (v_log_f32), v_sqrt_f32,
+  // and sin / cos of 2 pi u from v_sin_f32 / v_cos_f32, which take revolutions
+  constexpr float M2LN2 = -1.3862943611198906f;
+  const float r1 = __builtin_amdgcn_sqrtf(M2LN2 * __builtin_amdgcn_logf(u1));
+  const float r2 = __builtin_amdgcn_sqrtf(M2LN2 * __builtin_amdgcn_logf(u3));
+  z[0] = r1 * __builtin_amdgcn_cosf(u2);
+  z[1] = r1 * __builtin_amdgcn_sinf(u2);
+  z[2] = r2 * __builtin_amdgcn_cosf(u4);
+  z[3] = r2 * __builtin_amdgcn_sinf(u4);
 }
 
 // ---------------------------------------------------------------------------------
