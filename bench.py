@@ -54,9 +54,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_bwd_traffic_rgb1080p_150k.
 
 
 def measured_traffic():
-    """HBM bytes per k_render_bwd launch on the default workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values scaled by the same
-    counters' reading on a plain copy kernel of known size); None if absent."""
+    """HBM bytes of one raster adjoint (k_bwd_compact + k_render_bwd) on the default
+    workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter
+    values scaled by the same counters' reading on a plain copy kernel of known size);
+    None if absent."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     vals = {}
@@ -307,7 +308,8 @@ def main():
     roof = None   # the PyTorch path has no HIP raster kernel to report
     if k_ms:
         achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)", "bound": "mfma",
+        roof = {"kernel": "raster adjoint: k_bwd_compact + k_render_bwd (one captured graph)",
+                "bound": "mfma",
                 "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,
                 "pairs": pairs, "flop_per_pair": f_bwd}
@@ -320,6 +322,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] calibration copy of {args.calib_copy} MiB done", file=sys.stderr, flush=True)
+    peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9   # the step's own peak
     warm = None
     if not args.no_warmstart and args.workload == WORKLOAD:
         try:
@@ -338,8 +341,7 @@ def main():
            "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
                       "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
-                      "peak_hbm_gb": torch.cuda.max_memory_allocated(dev) / 1e9,
-                      "stage_ms_eager": stages},
+                      "peak_hbm_gb": peak_gb, "stage_ms_eager": stages},
            "roofline": roof, "cpu_baseline": cpu, "time_to_psnr": warm}
     print(json.dumps(out), flush=True)
 
