@@ -6,8 +6,8 @@ complex64 field, 3 wavelengths (639/515/473 nm), 8 um pitch, 2x padded BLASM
 glue -> BLASM -> multi-plane L2 + SSIM loss -> full adjoint -> Adan.
 
 Prints one JSON line (rank 0).  ``time_to_psnr`` is a reduced learned-warm-start study
-(16 training pairs from 400-iteration 1080p fits, 2 held-out 1080p scenes; ``--workload
-warmstart`` runs the full one):
+(8 training pairs from 400-iteration 1080p fits, 800 render-loss fine-tuning steps, 2
+held-out 1080p scenes; ``--workload warmstart`` runs the full one):
 speed-up of the warm start to the plain fit's final per-plane PSNR.  ``roofline`` describes the dominant kernel, timed
 with HIP events on its launch stream inside the timed region.  ``cpu_baseline``
 times the CPU oracle on a bounded sample of the same workload (see
@@ -218,7 +218,8 @@ def bench_warmstart(args, dev, quick=False):
     from cvqpg_amd import warmstart
     log = lambda m: print(f"[bench] warmstart {m}", file=sys.stderr, flush=True)
     if quick:
-        res = warmstart.experiment(dev, n_train=16, fit_iters=400, train_steps=600,
+        res = warmstart.experiment(dev, n_train=8, fit_iters=400, train_steps=300,
+                                   ft_scenes=8, ft_steps=800,
                                    eval_sets=(((1920, 1080), 2, 1000),), log=log,
                                    with_mean=False)
     else:

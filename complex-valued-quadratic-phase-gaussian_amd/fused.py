@@ -197,6 +197,13 @@ class FusedTrainer:
         for _, fn in self._stages():
             fn()
 
+    def loss_and_grads(self):
+        """The step without the optimiser: loss into ``self.loss`` and the raw gradients
+        into ``self.grads`` (eager, stream-ordered, no host synchronisation)."""
+        for name, fn in self._stages():
+            if name != "adan":
+                fn()
+
     def step_timed(self, events):
         """Eager step with a HIP event after every stage (events: dict name -> list)."""
         e0 = torch.cuda.Event(enable_timing=True)

@@ -13,14 +13,19 @@ one row per 16x16 tile (the tile's 4 x 256 pixels of RGB + depth), a 1x1 stem to
 channels, a U-Net over the tile grid with 4 down stages (2x2 average pooling, 3x3
 convolutions, 32-64-128-256-256 channels), 4 up stages (nearest upsampling, 3x3
 convolutions, additive skips) and a 1x1 head to 8 slots x 18 values per tile;
-3.56 M parameters.  Training: weighted squared error against per-tile encodings of the
-Gaussians that plain fits produce, Adam on FP32 master weights with bf16 copies.  The
-head's bias starts at the mean slot encoding of the training set, so the untrained
-network is the image-independent "mean start" and training learns deviations from it.
-Decoding takes a predicted phase only where its (cos, sin) vector is long enough to be
-a confident prediction (|v| >= 1/2) and the mean slot's phase elsewhere: phases of
-fitted Gaussians are largely unpredictable from the image, and a start whose uncertain
-phases are coherent converges much faster than one whose phases scatter.
+3.56 M parameters.  Training, Adam on FP32 master weights with bf16 copies:
+  1. supervised: weighted squared error against per-tile encodings of the Gaussians
+     that plain fits produce, from a head whose bias is the training set's mean slot
+     encoding (so the untrained network is the image-independent "mean start");
+  2. render-loss fine-tuning (``finetune``): the decoded slots are rendered by the fused
+     step and the fit's own loss is back-propagated through the renderer, the decode and
+     the network, so the network learns starts that already render close to the
+     multi-plane targets.
+The fine-tuned network's start uses every slot (``all_slot_params``).  For a network
+trained only on the supervised targets, ``warm_params`` decodes the slots it marks
+present and takes a predicted phase only where its (cos, sin) vector is long enough to
+be confident (|v| >= 1/2), the mean slot's phase elsewhere: phases of fitted Gaussians
+are largely unpredictable from the image, and scattered phases slow the fit down.
 
 Slot encoding (per Gaussian, tile origin (16 tx, 16 ty)):
   0-1   position in the tile, 2 (m - 16 t) / 16 - 1
@@ -438,11 +443,88 @@ def params_from_slots(out, wl, ref=None):
     return [torch.cat([p[idx], r[:N - k]], 0).contiguous() for p, r in zip(pred, rest)]
 
 
+def all_slot_params(out, wl):
+    """Raw parameters of a scene started from slot encodings [Ty, Tx, 144] using every
+    slot (the render-tuned network's start): the Ty*Tx*8 decoded Gaussians, then the
+    workload's random initial Gaussians up to N."""
+    W, H = wl.img_size
+    pred, _ = decode(out, W, H, wl.gaussians.channels)
+    from .model import PARAM_NAMES
+    N = wl.gaussians.means_2d.shape[0]
+    rest = [getattr(wl.gaussians, n).detach() for n in PARAM_NAMES]
+    k = min(N, pred[0].shape[0])
+    return [torch.cat([p[:k], r[:N - k]], 0).contiguous() for p, r in zip(pred, rest)]
+
+
 def warm_params(net, wl, ref=None):
     """Raw parameters of a warm-started scene: the network's Gaussians (up to 8 per tile,
     the slots it marks present), then the workload's own random initial Gaussians up to
     its count N."""
     return params_from_slots(net.forward(network_input(wl)[None])[0], wl, ref=ref)
+
+
+def finetune(net, wls, steps, lr=3e-4, log=None):
+    """Render-loss fine-tuning: for one scene per step, the network's slots are decoded
+    into the scene's first Ty*Tx*8 Gaussians (the rest keep their random start), the
+    fused step runs without its optimiser (cvqpg kernels: raster, BLASM, loss, adjoints),
+    the raw-parameter gradients flow back through the decode (torch autograd on the
+    decode's elementwise maps) into the network's output, and the network takes one Adam
+    step on its own kernels.  The objective is the fit's own loss at its first step, so the
+    network learns starts that already render close to the targets.  Returns the loss
+    history (every 50 steps)."""
+    from .fused import FusedTrainer
+    from .model import PARAM_NAMES
+    trs = [FusedTrainer(wl, num_itrs=1000, warmup_iters=0, inject_noise=False) for wl in wls]
+    inits = [[getattr(wl.gaussians, n).detach().clone() for n in PARAM_NAMES] for wl in wls]
+    imgs = [network_input(wl)[None].contiguous() for wl in wls]
+    hist = []
+    for k in range(steps):
+        i = k % len(trs)
+        tr, wl = trs[i], wls[i]
+        W, H = wl.img_size
+        out = net.forward(imgs[i])
+        o = out[0].detach().clone().requires_grad_()
+        params, _ = decode(o, W, H, wl.gaussians.channels)
+        m = params[0].shape[0]
+        with torch.no_grad():
+            for p, q, r in zip(tr.params, params, inits[i]):
+                p.copy_(torch.cat([q, r[m:]], 0))
+        tr.iteration.zero_()
+        tr.loss_and_grads()
+        gv = tr.grad_views()
+        torch.autograd.backward(params, [gv[n][:m] for n in PARAM_NAMES])
+        A = net.activations(1, *out.shape[1:3])
+        A["grad"]["out"].copy_(o.grad[None])
+        net.backward(A)
+        net.adam(lr * 0.5 * (1 + math.cos(math.pi * k / steps)))
+        if k % 50 == 0 or k == steps - 1:
+            hist.append((k, tr.loss.clone()))
+    out = [(k, float(v)) for k, v in hist]
+    if log:
+        log("render fine-tune loss " + " ".join(f"{k}:{v:.4f}" for k, v in out[::max(1, len(out) // 8)]))
+    return out
+
+
+def start_loss(start_fn, wls):
+    """Mean loss of the fit's first step (no optimiser step) from ``start_fn``'s starts:
+    the validation measure that chooses between candidate starts."""
+    from .fused import FusedTrainer
+    from .model import PARAM_NAMES
+    vals = []
+    for wl in wls:
+        init = start_fn(wl)
+        saved = [getattr(wl.gaussians, n).detach().clone() for n in PARAM_NAMES]
+        with torch.no_grad():
+            for n, p in zip(PARAM_NAMES, init):
+                getattr(wl.gaussians, n).data.copy_(p)
+        tr = FusedTrainer(wl, num_itrs=1000, warmup_iters=0, inject_noise=False)
+        tr.loss_and_grads()
+        vals.append(float(tr.loss))
+        with torch.no_grad():
+            for n, p in zip(PARAM_NAMES, saved):
+                getattr(wl.gaussians, n).data.copy_(p)
+        del tr
+    return float(np.mean(vals))
 
 
 # ---------------------------------------------------------------------- experiment
@@ -628,15 +710,25 @@ def mean_slots(ts, ws):
     return (t * w).sum(0) / w.sum(0).clamp(min=1)
 
 
-def experiment(device="cuda", train_size=(1920, 1080), n_train=32, fit_iters=600,
-               train_steps=1500, eval_sets=(((1920, 1080), 4, 2000), ((512, 512), 4, 1000)),
+def experiment(device="cuda", train_size=(1920, 1080), n_train=16, fit_iters=600,
+               train_steps=600, ft_size=(512, 512), ft_source=(1920, 1080), ft_scenes=16,
+               ft_steps=1500, ft_lr=3e-4, val_scenes=4,
+               eval_sets=(((1920, 1080), 4, 2000), ((512, 512), 4, 1000)),
                eval_every=25, seed=0, log=None, train_source=None, with_mean=True):
-    """The whole warm-start study on one GPU: training pairs from plain fits (windows of
-    ``train_size`` cut from ``train_source`` frames, or whole ``train_size`` frames when it
-    is None), network training, then time-to-PSNR of the warm start against the plain
-    start on held-out scenes, and of an image-independent start from the mean slot
-    encoding, which isolates what the network's conditioning on the image adds.  Returns
-    a dict."""
+    """The whole warm-start study on one GPU:
+
+    1. training pairs from plain fits (whole ``train_size`` frames, or windows of it cut
+       from ``train_source`` frames);
+    2. supervised training on the pairs' slot encodings, from a head biased to their mean;
+    3. render-loss fine-tuning (``finetune``) on ``ft_scenes`` scenes of ``ft_size``
+       (windows cut from ``ft_source`` frames, or whole frames when it is None);
+    4. validation: the first-step loss on ``val_scenes`` further scenes chooses between the
+       fine-tuned network (every slot) and the supervised one (present slots, gated
+       phases); the plain start stays the fallback and the judge;
+    5. time-to-PSNR on held-out scenes of the chosen warm start against the plain start,
+       and of the image-independent start from the mean slot encoding, which isolates
+       what the network's conditioning on the image adds.
+    Returns a dict."""
     import time as _time
     dev = torch.device(device)
     t0 = _time.perf_counter()
@@ -655,16 +747,46 @@ def experiment(device="cuda", train_size=(1920, 1080), n_train=32, fit_iters=600
     hist = train(net, xs, ts, ws, train_steps, lr=1e-3, seed=seed, log=log)
     torch.cuda.synchronize(dev)
     t_train = _time.perf_counter() - t1
+    del xs, ts, ws
+    sup = WarmStartNet(dev, seed=seed)
+    sup.load_state_dict(net.state_dict())
+
+    def scenes(n, salt):
+        if ft_source is None:
+            sp = scene_spec(*ft_size)
+            return [make_workload(sp, 30_000 + seed * 100_000 + salt * 1000 + i, dev)
+                    for i in range(n)]
+        sps, sds = crop_specs(ft_size, ft_source, n, seed + salt)
+        return [make_workload(sp, s, dev) for sp, s in zip(sps, sds)]
+
+    t2 = _time.perf_counter()
+    ft_hist = finetune(net, scenes(ft_scenes, 7), ft_steps, ft_lr, log=log) if ft_steps else []
+    torch.cuda.synchronize(dev)
+    t_ft = _time.perf_counter() - t2
+    val = scenes(val_scenes, 11)
+    tuned = lambda wl: all_slot_params(net.forward(network_input(wl)[None])[0], wl)
+    gated = lambda wl: warm_params(sup, wl, ref=tm)
+    loss_tuned = start_loss(tuned, val) if ft_steps else float("inf")
+    loss_gated = start_loss(gated, val)
+    choice = "fine-tuned" if loss_tuned <= loss_gated else "supervised"
+    if log:
+        log(f"validation first-step loss: fine-tuned {loss_tuned:.4f}, supervised {loss_gated:.4f}"
+            f" -> {choice}")
+    del val
     result = dict(network=dict(params=net.n_params, train_pairs=n_train,
                                train_frame=list(train_size),
                                train_source=list(train_source) if train_source else None,
-                               fit_iters=fit_iters,
-                               train_steps=train_steps, final_train_loss=hist[-1][1],
-                               pairs_s=t_pairs, train_s=t_train),
+                               fit_iters=fit_iters, train_steps=train_steps,
+                               final_train_loss=hist[-1][1], ft_frame=list(ft_size),
+                               ft_scenes=ft_scenes, ft_steps=ft_steps,
+                               ft_loss_first=ft_hist[0][1] if ft_hist else None,
+                               ft_loss_last=ft_hist[-1][1] if ft_hist else None,
+                               val_loss_finetuned=loss_tuned, val_loss_supervised=loss_gated,
+                               chosen=choice, pairs_s=t_pairs, train_s=t_train,
+                               finetune_s=t_ft),
                   evals=[])
-    del xs, ts, ws
     grid = lambda wl: tm.expand(-(-wl.img_size[1] // 16), -(-wl.img_size[0] // 16), OUT_DIM)
-    starts = {"net": lambda wl: warm_params(net, wl, ref=tm)}
+    starts = {"net": tuned if choice == "fine-tuned" else gated}
     if with_mean:
         starts["mean"] = lambda wl: params_from_slots(grid(wl), wl)
     for (W, H), n, iters in eval_sets:

@@ -1,12 +1,13 @@
 """What the learned warm start gains over the plain (random) start.
 
-A reduced version of ``warmstart.experiment``: 16 training pairs from 300-iteration
+A reduced version of ``warmstart.experiment``: 8 training pairs from 300-iteration
 plain fits of 256x256 RGB scenes (the 1080p recipe's Gaussian density and planes), 300
-network training steps, then two held-out scenes: the plain fit runs 400 iterations and
-its final per-plane PSNR is the bar; the warm-started fit (network Gaussians + random
-ones up to the same N) must reach every plane's bar minus 0.1 dB in at most half the
-time, and within one evaluation (25 iterations) of the image-independent start from the
-mean slot encoding.
+supervised steps, 400 render-loss fine-tuning steps on 8 further 256x256 scenes, the
+validation choice between the fine-tuned and the supervised network, then two held-out
+scenes: the plain fit runs 400 iterations and its final per-plane PSNR is
+the bar; the warm-started fit (network Gaussians + random ones up to the same N) must
+reach every plane's bar minus 0.1 dB in at most half the time, and within one evaluation
+(25 iterations) of the image-independent start from the mean slot encoding.
 """
 import pytest
 import torch
@@ -17,9 +18,10 @@ DEV = "cuda"
 
 def test_warm_start_reaches_plain_psnr_faster(pkg):
     from cvqpg_amd import warmstart
-    res = warmstart.experiment(DEV, train_size=(256, 256), n_train=16, fit_iters=300,
-                               train_steps=300, eval_sets=(((256, 256), 2, 400),),
-                               eval_every=25, seed=1, train_source=None,
+    res = warmstart.experiment(DEV, train_size=(256, 256), n_train=8, fit_iters=300,
+                               train_steps=300, ft_size=(256, 256), ft_source=None,
+                               ft_scenes=8, ft_steps=400,
+                               eval_sets=(((256, 256), 2, 400),), eval_every=25, seed=1,
                                log=lambda m: print(m, flush=True))
     e = res["evals"][0]
     for r in e["rows"]:
