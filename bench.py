@@ -8,7 +8,8 @@ glue -> BLASM -> multi-plane L2 + SSIM loss -> full adjoint -> Adan.
 Prints one JSON line (rank 0).  ``time_to_psnr`` is a reduced learned-warm-start study
 (8 training pairs from 400-iteration 1080p fits, 800 render-loss fine-tuning steps, 2
 held-out 1080p scenes; ``--workload warmstart`` runs the full one):
-speed-up of the warm start to the plain fit's final per-plane PSNR.  ``roofline`` describes the dominant kernel, timed
+speed-up of the warm start to the plain fit's final per-plane PSNR.  ``roofline_blasm`` gives
+the achieved HBM rate of the FFT passes.  ``roofline`` describes the dominant kernel, timed
 with HIP events on its launch stream inside the timed region.  ``cpu_baseline``
 times the CPU oracle on a bounded sample of the same workload (see
 ``cpu_baseline``).
@@ -53,21 +54,48 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_bwd_traffic_rgb1080p_150k.txt")
 
 
+BLASM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blasm_traffic_rgb1080p_150k.txt")
+
+
+def read_kv(path):
+    """key=value lines of a committed profile summary ('#' lines are comments)."""
+    vals = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                if "=" in line and not line.lstrip().startswith("#"):
+                    k, v = line.split("=", 1)
+                    vals[k.strip()] = v.strip()
+    return vals
+
+
+def blasm_roofline(stages, workload):
+    """HBM roofline of the BLASM passes (forward f1-f3 + adjoint a3-a1) from the eager
+    per-stage HIP-event times: the measured bytes of the committed rocprofv3 summary and
+    the algorithmic model of the unpruned schedule (30 two-dimensional FFTs of the 2x grid
+    at 32 B per point), both per step."""
+    vals = read_kv(BLASM_TRAFFIC_FILE)
+    if workload != WORKLOAD or "blasm_bytes_per_step" not in vals or not stages:
+        return None
+    ms = stages.get("blasm_fwd", 0.0) + stages.get("blasm_bwd", 0.0)
+    if ms <= 0:
+        return None
+    meas = float(vals["blasm_bytes_per_step"])
+    model = float(vals["blasm_model_bytes_per_step"])
+    return {"kernels": "k_f1/f2/f3 + k_a3/a2/a1 (BLASM forward and adjoint)", "bound": "hbm",
+            "achieved": model / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": meas,
+            "measured_gbs": meas / (ms * 1e-3) / 1e9, "ms": ms, "model_bytes": model,
+            "traffic_source": os.path.relpath(BLASM_TRAFFIC_FILE, ROOT)}
+
+
 def measured_traffic():
     """HBM bytes of one raster adjoint (k_bwd_compact + k_render_bwd) on the default
     workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter
     values scaled by the same counters' reading on a plain copy kernel of known size);
     None if absent."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None
-    vals = {}
-    with open(TRAFFIC_FILE) as f:
-        for line in f:
-            if "=" in line and not line.lstrip().startswith("#"):
-                k, v = line.split("=", 1)
-                vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_launch"])
+        return float(read_kv(TRAFFIC_FILE)["hbm_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
 
@@ -343,7 +371,8 @@ def main():
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
                       "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
                       "peak_hbm_gb": peak_gb, "stage_ms_eager": stages},
-           "roofline": roof, "cpu_baseline": cpu, "time_to_psnr": warm}
+           "roofline": roof, "roofline_blasm": blasm_roofline(stages, args.workload),
+           "cpu_baseline": cpu, "time_to_psnr": warm}
     print(json.dumps(out), flush=True)
 
 
