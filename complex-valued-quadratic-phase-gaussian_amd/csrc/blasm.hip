@@ -145,7 +145,9 @@ __device__ __forceinline__ float tf_point(const Optics& o, int c, int kr, int kc
   float fy, fx;
   tf_freqs(o, kr, kc, ir, ic, fy, fx);
   const float x = o.lam2[c] * fmaf(fx, fx, fy * fy);
-  float g = x <= 1.0f ? x / (1.0f + sqrtf(1.0f - x)) : -1.0f;
+  // g is a small number (x << 1 on these grids) whose product with z / lambda is a few
+  // revolutions, so the hardware square root and reciprocal (1 ulp) are ample
+  float g = x <= 1.0f ? x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - x)) : -1.0f;
   if (!o.ap_open) {
     const float X = fmaf((float)ir, o.aprs, o.apr0), Y = fmaf((float)ic, o.apcs, o.apc0);
     if (!(X * X + Y * Y < o.ap_r2)) g = -1.0f;
