@@ -400,6 +400,23 @@ __global__ void k_pack(const float2* __restrict__ means, const float* __restrict
   for (int i = 0; i < REC / 4; ++i) dst[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
 }
 
+// Maximum over a tile's pixel rectangle [x0, x1] x [y0, y1] of the concave exponent
+// q = a00 dx^2 + 2 a01 dx dy + a11 dy^2 (dx = mx - x, dy = my - y; the a's are the
+// packed, log2-scaled, negative-definite coefficients): the centre if it lies inside,
+// otherwise the largest of the four edge maxima.
+__device__ __forceinline__ float tile_qmax(float mx, float my, float a00, float a01, float a11,
+                                           float x0, float x1, float y0, float y1) {
+  const float dxlo = mx - x1, dxhi = mx - x0, dylo = my - y1, dyhi = my - y0;
+  if (dxlo <= 0.f && dxhi >= 0.f && dylo <= 0.f && dyhi >= 0.f) return 0.f;
+  auto qf = [&](float dx, float dy) { return fmaf(dx, fmaf(a00, dx, 2.f * a01 * dy), a11 * dy * dy); };
+  const float ry = -a01 / a11, rx = -a01 / a00;
+  float best = qf(dxlo, fminf(fmaxf(ry * dxlo, dylo), dyhi));
+  best = fmaxf(best, qf(dxhi, fminf(fmaxf(ry * dxhi, dylo), dyhi)));
+  best = fmaxf(best, qf(fminf(fmaxf(rx * dylo, dxlo), dxhi), dylo));
+  best = fmaxf(best, qf(fminf(fmaxf(rx * dyhi, dxlo), dxhi), dyhi));
+  return best;
+}
+
 // ------------------------------------------------------------------------------
 // raster forward
 // ------------------------------------------------------------------------------
@@ -412,38 +429,55 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
                                                 const int32_t* __restrict__ point_list,
                                                 const float4* __restrict__ packed,
                                                 const uint32_t* __restrict__ misc, int W, int H,
-                                                int gx, float2* __restrict__ out, float4* sg) {
+                                                int gx, float2* __restrict__ out, float4* sg,
+                                                int* s_cnt) {
   const int tx = tile % gx, ty = tile / gx;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int px = tx * TILE + (t & 15);
   const int py = ty * TILE + (t >> 4);
   int2 rg = ranges[tile];
   if (misc[1]) rg = make_int2(0, 0);
   const float fpx = (float)px, fpy = (float)py;
+  const float tx0 = (float)(tx * TILE), ty0 = (float)(ty * TILE);
+  const float tx1 = (float)(min(tx * TILE + TILE, W) - 1), ty1 = (float)(min(ty * TILE + TILE, H) - 1);
   float ar[C], ai[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) ar[c] = ai[c] = 0.f;
   for (int b0 = rg.x; b0 < rg.y; b0 += 256) {
     const int cnt = min(256, rg.y - b0);
     __syncthreads();
+    // stage the batch's Gaussians, keeping (in key order) only those that reach
+    // G = 2^-24 somewhere on the tile's image pixels: the others would be skipped by
+    // every wave below
+    float4 q0, q1, q2, q3;
+    bool keep = false;
     if (t < cnt) {
       const int g = point_list[b0 + t];
       const float4* src = packed + (size_t)g * (REC / 4);
-      const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
+      q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
+      keep = tile_qmax(q0.x, q0.y, q0.z, q0.w, q1.x, tx0, tx1, ty0, ty1) >= QSKIP;
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) s_cnt[wave] = __popcll(bal);
+    __syncthreads();
+    int slot = __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) slot += s_cnt[w];
+    const int kept = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    if (keep) {
       // q0 = mx my a00 a01 ; q1 = a11 o beta u0 ; q2 = w0 col0 u1 w1 ; q3 = col1 u2 w2 col2
       const float o = q1.y;
       if (C == 3) {
-        sg[4 * t + 0] = q0;
-        sg[4 * t + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
-        sg[4 * t + 2] = make_float4(q2.z, q2.w, q3.x * o, q3.y);
-        sg[4 * t + 3] = make_float4(q3.z, q3.w * o, 0.f, 0.f);
+        sg[4 * slot + 0] = q0;
+        sg[4 * slot + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
+        sg[4 * slot + 2] = make_float4(q2.z, q2.w, q3.x * o, q3.y);
+        sg[4 * slot + 3] = make_float4(q3.z, q3.w * o, 0.f, 0.f);
       } else {
-        sg[4 * t + 0] = q0;
-        sg[4 * t + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
+        sg[4 * slot + 0] = q0;
+        sg[4 * slot + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
       }
     }
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
+    for (int j = 0; j < kept; ++j) {
       const float4 p0 = sg[4 * j + 0];
       const float4 p1 = sg[4 * j + 1];
       const float dx = p0.x - fpx, dy = p0.y - fpy;
@@ -489,7 +523,8 @@ __global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ran
                                                     const uint32_t* __restrict__ misc, int W,
                                                     int H, int gx, float2* __restrict__ out) {
   __shared__ float4 sg[256 * 4];
-  render_fwd_tile<C>(blockIdx.x, ranges, point_list, packed, misc, W, H, gx, out, sg);
+  __shared__ int s_cnt[4];
+  render_fwd_tile<C>(blockIdx.x, ranges, point_list, packed, misc, W, H, gx, out, sg, s_cnt);
 }
 
 // ------------------------------------------------------------------------------
@@ -663,21 +698,7 @@ __global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ran
 // 2^-24 of a unit term to every adjoint sum (the reference evaluates it, down to its
 // clamp at e^-50).  Per tile, the keys that reach 2^-24 somewhere are copied, in key
 // order, to the front of the tile's range in klist and counted in kcount; the partial
-// records of the others are zeroed so the per-Gaussian sums stay complete.  The maximum
-// of the concave exponent q over the tile's pixel rectangle is exact: the centre if it
-// lies inside, otherwise the largest of the four edge maxima.
-__device__ __forceinline__ float tile_qmax(float mx, float my, float a00, float a01, float a11,
-                                           float x0, float x1, float y0, float y1) {
-  const float dxlo = mx - x1, dxhi = mx - x0, dylo = my - y1, dyhi = my - y0;
-  if (dxlo <= 0.f && dxhi >= 0.f && dylo <= 0.f && dyhi >= 0.f) return 0.f;
-  auto qf = [&](float dx, float dy) { return fmaf(dx, fmaf(a00, dx, 2.f * a01 * dy), a11 * dy * dy); };
-  const float ry = -a01 / a11, rx = -a01 / a00;
-  float best = qf(dxlo, fminf(fmaxf(ry * dxlo, dylo), dyhi));
-  best = fmaxf(best, qf(dxhi, fminf(fmaxf(ry * dxhi, dylo), dyhi)));
-  best = fmaxf(best, qf(fminf(fmaxf(rx * dylo, dxlo), dxhi), dylo));
-  best = fmaxf(best, qf(fminf(fmaxf(rx * dyhi, dxlo), dxhi), dyhi));
-  return best;
-}
+// records of the others are zeroed so the per-Gaussian sums stay complete.
 
 __device__ __forceinline__ void compact_tile(int tile, const int2* __restrict__ ranges,
                                              const int32_t* __restrict__ point_list,
@@ -799,6 +820,7 @@ __global__ __launch_bounds__(256) void k_render_fwd_batch(BatchHeader* __restric
   __shared__ float4 sg[256 * 4];
   __shared__ int begin[MAX_BATCH];
   __shared__ int slot;
+  __shared__ int s_cnt[4];
   const int ns = hdr->n_scenes, total = hdr->total_tiles;
   for (int i = threadIdx.x; i < ns; i += 256) begin[i] = sc[i].tile_begin;
   for (;;) {
@@ -810,7 +832,7 @@ __global__ __launch_bounds__(256) void k_render_fwd_batch(BatchHeader* __restric
       s = batch_scene_of(begin, ns, it, s);
       const BatchScene& S = sc[s];
       render_fwd_tile<C>(it - S.tile_begin, S.ranges, S.point_list, S.packed, S.misc, S.W, S.H,
-                         S.gx, S.field, sg);
+                         S.gx, S.field, sg, s_cnt);
     }
   }
   batch_exit(hdr, 0);
