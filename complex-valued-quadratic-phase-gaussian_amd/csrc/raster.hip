@@ -74,20 +74,12 @@ struct RasterWS {
   uint32_t* hist;       // [256 * nb]
   uint32_t* sums_h;     // scan block sums of the histogram scans
   float* partials;      // [cap][REC]
-  int32_t* klist;       // [cap]      per tile, the keys the adjoint evaluates (a prefix of
-                        //            the tile's range, in key order)
-  // tile-count-dependent region
-  int32_t* kcount;      // [tiles]    length of each tile's prefix in klist
   size_t bytes;
 };
 
 static int sort_blocks(int64_t cap) { return ceil_div(cap > 0 ? cap : 1, SORT_BLOCK); }
 
-static int tiles_of(int width, int height) {
-  return ceil_div(width > 0 ? width : 1, TILE) * ceil_div(height > 0 ? height : 1, TILE);
-}
-
-static RasterWS carve(void* base, int n, int64_t cap, int tiles) {
+static RasterWS carve(void* base, int n, int64_t cap) {
   Carver cv(base);
   RasterWS w;
   int nn = n > 0 ? n : 1;
@@ -106,8 +98,6 @@ static RasterWS carve(void* base, int n, int64_t cap, int tiles) {
   w.hist = cv.take<uint32_t>((size_t)256 * nb);
   w.sums_h = cv.take<uint32_t>(ceil_div((int64_t)256 * nb, SCAN_BLOCK) + 1);
   w.partials = cv.take<float>((size_t)cc * REC);
-  w.klist = cv.take<int32_t>(cc);
-  w.kcount = cv.take<int32_t>(tiles > 0 ? tiles : 1);
   w.bytes = cv.bytes();
   return w;
 }
@@ -531,22 +521,29 @@ __global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ran
 // raster adjoint: one Gaussian per lane, pixels as wave-uniform broadcasts
 // ------------------------------------------------------------------------------
 // partial record (16 floats): dcol[C], dph'[C], dop, K', di00, di01', di11, dmx', dmy'
-// One tile of one hologram over the keys klist[rg.x, rg.y) (the tile's compacted keys);
-// sgr is the workgroup's [C][256] pixel-gradient buffer.  The first write to sgr is not
-// preceded by a barrier here: a workgroup that calls this for several tiles places one
-// between the calls.
+// One tile of one hologram; sgr is the workgroup's [C][256] pixel-gradient buffer,
+// s_keys a 512-entry queue of keys and s_cnt 4 counters (LDS).  A key whose Gaussian
+// stays below G = 2^-24 on every pixel of the tile adds less than 2^-24 of a unit term
+// to every adjoint sum (the reference evaluates it, down to its clamp at e^-50): such
+// keys get a zero partial record (so the per-Gaussian sums stay complete) and are not
+// evaluated.  The others are queued in key order and evaluated 256 at a time, one per
+// lane.  The first write to sgr is not preceded by a barrier here: a workgroup that calls
+// this for several tiles places one between the calls.
 template <int C>
-__device__ __forceinline__ void render_bwd_tile(int tile, int2 rg,
+__device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict__ ranges,
                                                 const int32_t* __restrict__ point_list,
                                                 const float4* __restrict__ packed,
                                                 const int4* __restrict__ rect,
                                                 const float2* __restrict__ grad,
-                                                const float* __restrict__ kvec, int W, int H,
+                                                const float* __restrict__ kvec,
+                                                const uint32_t* __restrict__ misc, int W, int H,
                                                 int gx, float4* __restrict__ partials,
-                                                float2 (*sgr)[256]) {
+                                                float2 (*sgr)[256], int* s_keys, int* s_cnt) {
   const int tx = tile % gx, ty = tile / gx;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
+  int2 rg = ranges[tile];
+  if (misc[1]) rg = make_int2(0, 0);
   if (rg.y <= rg.x) return;  // uniform
   {
     const int px = tx * TILE + (t & 15), py = ty * TILE + (t >> 4);
@@ -562,10 +559,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, int2 rg,
 #pragma unroll
   for (int c = 0; c < C; ++c) kc[c] = kvec[c];
 
-  for (int base = rg.x + wave * 64; base < rg.y; base += 256) {
-    const int slot = base + lane;
-    const bool valid = slot < rg.y;
-    const int g = point_list[valid ? slot : rg.x];
+  auto process = [&](int g, bool valid) {
     const float4* src = packed + (size_t)g * (REC / 4);
     const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
     const float mx = q0.x, my = q0.y, a00 = q0.z, a01 = q0.w, a11 = q1.x;
@@ -672,92 +666,68 @@ __device__ __forceinline__ void render_bwd_tile(int tile, int2 rg,
       for (int i = 0; i < REC / 4; ++i)
         dst[i] = make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
     }
-  }
-}
+  };
 
-template <int C>
-__global__ __launch_bounds__(256) void k_render_bwd(const int2* __restrict__ ranges,
-                                                    const int32_t* __restrict__ klist,
-                                                    const int32_t* __restrict__ kcount,
-                                                    const float4* __restrict__ packed,
-                                                    const int4* __restrict__ rect,
-                                                    const float2* __restrict__ grad,
-                                                    const float* __restrict__ kvec, int W,
-                                                    int H, int gx, float4* __restrict__ partials) {
-  __shared__ float2 sgr[C][256];
-  const int tile = blockIdx.x;
-  const int start = ranges[tile].x;
-  render_bwd_tile<C>(tile, make_int2(start, start + kcount[tile]), klist, packed, rect, grad,
-                     kvec, W, H, gx, partials, sgr);
-}
-
-// ------------------------------------------------------------------------------
-// adjoint key compaction
-// ------------------------------------------------------------------------------
-// A key whose Gaussian stays below G = 2^-24 on every pixel of its tile adds less than
-// 2^-24 of a unit term to every adjoint sum (the reference evaluates it, down to its
-// clamp at e^-50).  Per tile, the keys that reach 2^-24 somewhere are copied, in key
-// order, to the front of the tile's range in klist and counted in kcount; the partial
-// records of the others are zeroed so the per-Gaussian sums stay complete.
-
-__device__ __forceinline__ void compact_tile(int tile, const int2* __restrict__ ranges,
-                                             const int32_t* __restrict__ point_list,
-                                             const float4* __restrict__ packed,
-                                             const int4* __restrict__ rect,
-                                             const uint32_t* __restrict__ misc, int W, int H,
-                                             int gx, int32_t* __restrict__ klist,
-                                             int32_t* __restrict__ kcount,
-                                             float4* __restrict__ partials, int* s_cnt) {
-  const int tx = tile % gx, ty = tile / gx;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  int2 rg = ranges[tile];
-  if (misc[1]) rg = make_int2(0, 0);
-  const float x0 = (float)(tx * TILE), y0 = (float)(ty * TILE);
-  const float x1 = (float)(min(tx * TILE + TILE, W) - 1), y1 = (float)(min(ty * TILE + TILE, H) - 1);
-  int total = 0;
+  const float fx0 = (float)x0, fy0 = (float)y0;
+  const float fx1 = (float)(x0 + nx - 1), fy1 = (float)(y0 + ny - 1);
+  int filled = 0;  // queued keys in s_keys[0, filled)
   for (int base = rg.x; base < rg.y; base += 256) {
     const int i = base + t;
-    const bool valid = i < rg.y;
-    int g = 0;
     bool keep = false;
-    if (valid) {
+    int g = 0;
+    if (i < rg.y) {
       g = point_list[i];
       const float4 q0 = packed[(size_t)g * (REC / 4)];
       const float a11 = packed[(size_t)g * (REC / 4) + 1].x;
-      keep = tile_qmax(q0.x, q0.y, q0.z, q0.w, a11, x0, x1, y0, y1) >= QSKIP;
+      keep = tile_qmax(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, fy1) >= QSKIP;
+      if (!keep) {
+        const int4 rc = rect[g];
+        const int e = rc.x + (ty - rc.z) * rc.w + (tx - rc.y);
+        float4* dst = partials + (size_t)e * (REC / 4);
+#pragma unroll
+        for (int k = 0; k < REC / 4; ++k) dst[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
     const uint64_t bal = __ballot(keep);
     if (lane == 0) s_cnt[wave] = __popcll(bal);
     __syncthreads();
-    int off = total;
-    for (int w = 0; w < wave; ++w) off += s_cnt[w];
-    const int blk = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    if (keep) {
-      klist[rg.x + off + __popcll(bal & ((1ull << lane) - 1ull))] = g;
-    } else if (valid) {
-      const int4 rc = rect[g];
-      const int e = rc.x + (ty - rc.z) * rc.w + (tx - rc.y);
-      float4* dst = partials + (size_t)e * (REC / 4);
-#pragma unroll
-      for (int k = 0; k < REC / 4; ++k) dst[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    int pos = filled + __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) pos += s_cnt[w];
+    if (keep) s_keys[pos] = g;
+    filled += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    __syncthreads();  // the queue is complete; s_cnt may be rewritten from here on
+    const bool last = base + 256 >= rg.y;
+    while (filled >= 256 || (last && filled > 0)) {
+      const int n = min(filled, 256);
+      if (wave * 64 < n) {
+        const int slot = wave * 64 + lane;
+        process(s_keys[slot < n ? slot : 0], slot < n);
+      }
+      __syncthreads();  // every lane has read its key
+      if (t < filled - 256) s_keys[t] = s_keys[256 + t];
+      filled -= n;
+      __syncthreads();
     }
-    total += blk;
-    __syncthreads();  // s_cnt is rewritten by the next round
   }
-  if (t == 0) kcount[tile] = total;
 }
 
-__global__ __launch_bounds__(256) void k_bwd_compact(const int2* __restrict__ ranges,
-                                                     const int32_t* __restrict__ point_list,
-                                                     const float4* __restrict__ packed,
-                                                     const int4* __restrict__ rect,
-                                                     const uint32_t* __restrict__ misc, int W,
-                                                     int H, int gx, int32_t* __restrict__ klist,
-                                                     int32_t* __restrict__ kcount,
-                                                     float4* __restrict__ partials) {
+#ifndef CVQPG_BWD_WPE
+#define CVQPG_BWD_WPE 5
+#endif
+template <int C>
+__global__ __launch_bounds__(256, CVQPG_BWD_WPE) void k_render_bwd(const int2* __restrict__ ranges,
+                                                    const int32_t* __restrict__ point_list,
+                                                    const float4* __restrict__ packed,
+                                                    const int4* __restrict__ rect,
+                                                    const float2* __restrict__ grad,
+                                                    const float* __restrict__ kvec,
+                                                    const uint32_t* __restrict__ misc, int W,
+                                                    int H, int gx, float4* __restrict__ partials) {
+  __shared__ float2 sgr[C][256];
+  __shared__ int s_keys[512];
   __shared__ int s_cnt[4];
-  compact_tile(blockIdx.x, ranges, point_list, packed, rect, misc, W, H, gx, klist, kcount,
-               partials, s_cnt);
+  render_bwd_tile<C>(blockIdx.x, ranges, point_list, packed, rect, grad, kvec, misc, W, H, gx,
+                     partials, sgr, s_keys, s_cnt);
 }
 
 // ------------------------------------------------------------------------------
@@ -779,8 +749,6 @@ struct BatchScene {
   const float2* grad;
   const float* k;
   float4* partials;
-  int32_t* klist;
-  int32_t* kcount;
   int W, H, gx, tile_begin;
 };
 
@@ -838,26 +806,13 @@ __global__ __launch_bounds__(256) void k_render_fwd_batch(BatchHeader* __restric
   batch_exit(hdr, 0);
 }
 
-// the adjoint key compaction of every scene: the grid strides over all tiles of the batch
-__global__ __launch_bounds__(256) void k_bwd_compact_batch(const BatchHeader* __restrict__ hdr,
-                                                           const BatchScene* __restrict__ sc) {
-  __shared__ int s_cnt[4];
-  const int ns = hdr->n_scenes, total = hdr->total_tiles;
-  int s = 0;
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    while (s + 1 < ns && item >= sc[s + 1].tile_begin) ++s;
-    s = __builtin_amdgcn_readfirstlane(s);
-    const BatchScene& S = sc[s];
-    compact_tile(item - S.tile_begin, S.ranges, S.point_list, S.packed, S.rect, S.misc, S.W,
-                 S.H, S.gx, S.klist, S.kcount, S.partials, s_cnt);
-  }
-}
-
 template <int C>
-__global__ __launch_bounds__(256, 5) void k_render_bwd_batch(BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(256, CVQPG_BWD_WPE) void k_render_bwd_batch(BatchHeader* __restrict__ hdr,
                                                           const BatchScene* __restrict__ sc,
                                                           int chunk) {
   __shared__ float2 sgr[C][256];
+  __shared__ int s_keys[512];
+  __shared__ int s_cnt[4];
   __shared__ int begin[MAX_BATCH];
   __shared__ int slot;
   const int ns = hdr->n_scenes, total = hdr->total_tiles;
@@ -871,10 +826,8 @@ __global__ __launch_bounds__(256, 5) void k_render_bwd_batch(BatchHeader* __rest
       if (it > first) __syncthreads();  // the previous tile's waves are done with sgr
       s = batch_scene_of(begin, ns, it, s);
       const BatchScene& S = sc[s];
-      const int tile = it - S.tile_begin;
-      const int start = S.ranges[tile].x;
-      render_bwd_tile<C>(tile, make_int2(start, start + S.kcount[tile]), S.klist, S.packed,
-                         S.rect, S.grad, S.k, S.W, S.H, S.gx, S.partials, sgr);
+      render_bwd_tile<C>(it - S.tile_begin, S.ranges, S.point_list, S.packed, S.rect, S.grad,
+                         S.k, S.misc, S.W, S.H, S.gx, S.partials, sgr, s_keys, s_cnt);
     }
   }
   batch_exit(hdr, 1);
@@ -1165,7 +1118,7 @@ static int check_common(int n, int channels, int width, int height, void* ws, si
   CVQPG_CHECK_ARG(capacity >= 0 && capacity < (int64_t)1 << 31, "invalid capacity %lld",
                   (long long)capacity);
   CVQPG_CHECK_ARG(ws != nullptr, "workspace is NULL");
-  const size_t need = carve(nullptr, n, capacity, tiles_of(width, height)).bytes;
+  const size_t need = carve(nullptr, n, capacity).bytes;
   CVQPG_CHECK_ARG(ws_bytes >= need, "workspace too small: %zu < %zu bytes", ws_bytes, need);
   return 0;
 }
@@ -1177,7 +1130,9 @@ using namespace cvqpg;
 extern "C" size_t cvqpg_raster_workspace_size(int n, int channels, int width, int height,
                                               int64_t capacity) {
   (void)channels;
-  return carve(nullptr, n, capacity, tiles_of(width, height)).bytes;
+  (void)width;
+  (void)height;
+  return carve(nullptr, n, capacity).bytes;
 }
 
 extern "C" int cvqpg_bin_count(const float* means, const int32_t* radii, int n, int width,
@@ -1185,7 +1140,7 @@ extern "C" int cvqpg_bin_count(const float* means, const int32_t* radii, int n, 
                                int32_t* num_pairs, void* stream) {
   if (int rc = check_common(n, 1, width, height, ws, ws_bytes, capacity)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   if (n > 0) {
     CVQPG_CHECK_ARG(means && radii, "means/radii are NULL");
@@ -1204,7 +1159,7 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   if (int rc = check_common(n, 1, width, height, ws, ws_bytes, capacity)) return rc;
   CVQPG_CHECK_ARG(point_list && ranges, "point_list/ranges are NULL");
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   const int T = gx * gy;
   const int64_t cap = capacity > 0 ? capacity : 1;
@@ -1264,7 +1219,7 @@ extern "C" int cvqpg_raster_pack(const float* means, const float* inv_cov, const
   CVQPG_CHECK_ARG(means && inv_cov && colours && phase && opacities && curvature && area && k,
                   "NULL Gaussian parameter pointer");
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   if (channels == 3)
     launch_pack<3>(means, inv_cov, colours, phase, opacities, curvature, area, k, dxy2, n, w.packed, s);
   else
@@ -1280,7 +1235,7 @@ extern "C" int cvqpg_render_forward_packed(int n, int channels, int width, int h
   if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
   CVQPG_CHECK_ARG(field && ranges && point_list, "NULL field/ranges/point_list");
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   if (channels == 3)
     hipLaunchKernelGGL(k_render_fwd<3>, dim3(gx * gy), dim3(256), 0, s,
@@ -1305,21 +1260,18 @@ extern "C" int cvqpg_render_backward_tiles(const float* grad_field, const float*
   if (n == 0) return 0;
   CVQPG_CHECK_ARG(grad_field && ranges && point_list && k, "NULL grad/ranges/point_list/k");
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   const int2* rg = reinterpret_cast<const int2*>(ranges);
   const float4* pk = reinterpret_cast<const float4*>(w.packed);
+  const float2* gf = reinterpret_cast<const float2*>(grad_field);
   float4* part = reinterpret_cast<float4*>(w.partials);
-  hipLaunchKernelGGL(k_bwd_compact, dim3(gx * gy), dim3(256), 0, s, rg, point_list, pk, w.rect,
-                     w.misc, width, height, gx, w.klist, w.kcount, part);
   if (channels == 3)
-    hipLaunchKernelGGL(k_render_bwd<3>, dim3(gx * gy), dim3(256), 0, s, rg, w.klist, w.kcount, pk,
-                       w.rect, reinterpret_cast<const float2*>(grad_field), k, width, height, gx,
-                       part);
+    hipLaunchKernelGGL(k_render_bwd<3>, dim3(gx * gy), dim3(256), 0, s, rg, point_list, pk, w.rect,
+                       gf, k, w.misc, width, height, gx, part);
   else
-    hipLaunchKernelGGL(k_render_bwd<1>, dim3(gx * gy), dim3(256), 0, s, rg, w.klist, w.kcount, pk,
-                       w.rect, reinterpret_cast<const float2*>(grad_field), k, width, height, gx,
-                       part);
+    hipLaunchKernelGGL(k_render_bwd<1>, dim3(gx * gy), dim3(256), 0, s, rg, point_list, pk, w.rect,
+                       gf, k, w.misc, width, height, gx, part);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }
@@ -1348,7 +1300,7 @@ extern "C" int cvqpg_raster_batch_table(const cvqpg_raster_scene* scenes, int n_
                     i, q.channels, scenes[0].channels);
     CVQPG_CHECK_ARG(q.point_list && q.ranges && q.field, "scene %d: NULL point_list/ranges/field",
                     i);
-    RasterWS w = carve(q.ws, q.n, q.capacity, tiles_of(q.width, q.height));
+    RasterWS w = carve(q.ws, q.n, q.capacity);
     const int gx = ceil_div(q.width, TILE), gy = ceil_div(q.height, TILE);
     BatchScene& b = sc[i];
     b.ranges = reinterpret_cast<const int2*>(q.ranges);
@@ -1360,8 +1312,6 @@ extern "C" int cvqpg_raster_batch_table(const cvqpg_raster_scene* scenes, int n_
     b.grad = reinterpret_cast<const float2*>(q.grad_field);
     b.k = q.k;
     b.partials = reinterpret_cast<float4*>(w.partials);
-    b.klist = w.klist;
-    b.kcount = w.kcount;
     b.W = q.width;
     b.H = q.height;
     b.gx = gx;
@@ -1403,7 +1353,6 @@ extern "C" int cvqpg_render_backward_batch(void* table, int channels, int grid, 
   hipStream_t s = (hipStream_t)stream;
   BatchHeader* hdr = static_cast<BatchHeader*>(table);
   const BatchScene* sc = reinterpret_cast<const BatchScene*>(hdr + 1);
-  hipLaunchKernelGGL(k_bwd_compact_batch, dim3(grid), dim3(256), 0, s, hdr, sc);
   if (channels == 3)
     hipLaunchKernelGGL(k_render_bwd_batch<3>, dim3(grid), dim3(256), 0, s, hdr, sc, chunk);
   else
@@ -1425,7 +1374,7 @@ extern "C" int cvqpg_render_backward_reduce(const float* colours, const float* c
                       d_area,
                   "NULL gradient output pointer");
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const dim3 gb(ceil_div(n, 256)), tb(256);
   if (channels == 3)
     hipLaunchKernelGGL(k_bwd_reduce<3>, gb, tb, 0, s, reinterpret_cast<const float4*>(w.partials),
@@ -1523,7 +1472,7 @@ extern "C" int cvqpg_prologue_forward(const float* const* raw_params, int n, int
   Prologue pr;
   if (int rc = make_prologue(pr, width, height, dxy2, curv_lambda, curv_mode, curv_nyquist)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const RawPtrs R = raw_ptrs(raw_params);
   if (channels == 3)
     hipLaunchKernelGGL(k_prologue<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k, n,
@@ -1548,7 +1497,7 @@ extern "C" int cvqpg_raster_backward_raw(const float* const* raw_params, int n, 
   Prologue pr;
   if (int rc = make_prologue(pr, width, height, dxy2, curv_lambda, curv_mode, curv_nyquist)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  RasterWS w = carve(ws, n, capacity, tiles_of(width, height));
+  RasterWS w = carve(ws, n, capacity);
   const RawPtrs R = raw_ptrs(raw_params);
   if (channels == 3)
     hipLaunchKernelGGL(k_bwd_reduce_raw<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s,

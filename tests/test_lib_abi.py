@@ -77,7 +77,7 @@ def test_batch_table_layout_and_errors(pkg):
         s.point_list, s.ranges, s.field = s.ws + 0x1000_0000, s.ws + 0x2000_0000, s.ws + 0x3000_0000
         s.grad_field, s.k = s.ws + 0x4000_0000, s.ws + 0x5000_0000
     nb = L.cvqpg_raster_batch_table_size(3)
-    assert nb == 64 + 3 * 104
+    assert nb == 64 + 3 * 88
     buf = (ctypes.c_uint8 * nb)()
     _lib.check(L.cvqpg_raster_batch_table(scenes, 3, buf, nb))
     raw = bytes(buf)
@@ -87,14 +87,13 @@ def test_batch_table_layout_and_errors(pkg):
     assert hdr[4] == 3 and hdr[5] == sum(tiles)
     begin = 0
     for i, (W, H, n) in enumerate(sizes):
-        rec = raw[64 + 104 * i: 64 + 104 * (i + 1)]
-        ptrs = np.frombuffer(rec[:88], dtype=np.uint64)
-        ints = np.frombuffer(rec[88:], dtype=np.int32)
+        rec = raw[64 + 88 * i: 64 + 88 * (i + 1)]
+        ptrs = np.frombuffer(rec[:72], dtype=np.uint64)
+        ints = np.frombuffer(rec[72:], dtype=np.int32)
         s = scenes[i]
-        # ranges, point_list, packed (= workspace base), rect, misc, field, grad, k, partials,
-        # compacted key list, compacted counts
+        # ranges, point_list, packed (= workspace base), rect, misc, field, grad, k, partials
         assert ptrs[0] == s.ranges and ptrs[1] == s.point_list and ptrs[2] == s.ws
-        assert s.ws < ptrs[3] < ptrs[4] < ptrs[8] < ptrs[9] < ptrs[10] < s.ws + s.ws_bytes
+        assert s.ws < ptrs[3] < ptrs[4] < ptrs[8] < s.ws + s.ws_bytes
         assert ptrs[5] == s.field and ptrs[6] == s.grad_field and ptrs[7] == s.k
         assert list(ints) == [W, H, (W + 15) // 16, begin]
         begin += tiles[i]
