@@ -90,10 +90,9 @@ def blasm_roofline(stages, workload):
 
 
 def measured_traffic():
-    """HBM bytes of one raster adjoint (k_bwd_compact + k_render_bwd) on the default
-    workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter
-    values scaled by the same counters' reading on a plain copy kernel of known size);
-    None if absent."""
+    """HBM bytes per k_render_bwd launch on the default workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values scaled by the same
+    counters' reading on a plain copy kernel of known size); None if absent."""
     try:
         return float(read_kv(TRAFFIC_FILE)["hbm_bytes_per_launch"])
     except (KeyError, ValueError):
@@ -337,7 +336,7 @@ def main():
     roof = None   # the PyTorch path has no HIP raster kernel to report
     if k_ms:
         achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
-        roof = {"kernel": "raster adjoint: k_bwd_compact + k_render_bwd (one captured graph)",
+        roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)",
                 "bound": "mfma",
                 "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,

@@ -100,8 +100,8 @@ int cvqpg_render_backward(const float *grad_field, const float *means, const flo
  *   forward_packed  renderTileKernel<C> on packed parameters (cuda/forward.cu:62-162);
  *   backward_tiles  the per-tile adjoint kernel writing one partial record per key
  *                   (cuda/backward.cu:22-148 without its atomics, :150-165); keys whose
- *                   Gaussian stays below G = 2^-24 on the whole tile are left out (their
- *                   records are zero: each term is below 2^-24 of a unit term);
+ *                   Gaussian stays below G = 2^-24 on the whole tile are not evaluated
+ *                   (their records are zero: each term is below 2^-24 of a unit term);
  *   backward_reduce the per-Gaussian sum of the records and the final gradients. */
 int cvqpg_raster_pack(const float *means, const float *inv_cov, const float *colours,
                       const float *phase, const float *opacities, const float *curvature,
