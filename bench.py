@@ -16,7 +16,6 @@ times the CPU oracle on a bounded sample of the same workload (see
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -113,12 +112,11 @@ def cpu_baseline(spec, seed=0):
 
     Sample: (a) the reference's PyTorch tile renderer (3 sigma, 64-px tiles,
     oracle.model.render_tile_torch) forward + autograd backward in FP32 on a
-    256 x 256 window holding the Gaussians whose centres fall inside it, scaled by
+    512 x 512 window holding the Gaussians whose centres fall inside it, scaled by
     the frame/window area ratio; (b) one forward + one inverse complex64 2D FFT of
     the full 2816 x 4480 grid (torch.fft / pocketfft), scaled by the 4 D C FFTs of
     the reference step (forward and autograd backward).  Elementwise work of the
     step is not counted, so the baseline is optimistic for the CPU."""
-    import numpy as np
     import torch
 
     sys.path.insert(0, ROOT)

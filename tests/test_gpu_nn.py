@@ -8,7 +8,6 @@ rounding of the pre-activation gradients is the GPU's only other approximation).
 """
 import math
 
-import numpy as np
 import pytest
 import torch
 

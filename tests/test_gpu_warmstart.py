@@ -10,7 +10,6 @@ reach every plane's bar minus 0.1 dB in at most half the time, and within one ev
 (25 iterations) of the image-independent start from the mean slot encoding.
 """
 import pytest
-import torch
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
