@@ -19,8 +19,6 @@ import math
 
 import torch
 
-from . import _lib
-
 
 def freq_grid(n, dx, grid="odak", device="cpu"):
     X = dx * float(n)

@@ -11,7 +11,6 @@
 """
 import math
 
-import numpy as np
 import torch
 
 from . import loss as loss_mod

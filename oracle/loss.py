@@ -26,7 +26,6 @@ gradient against FP64 central finite differences.
 """
 import math
 
-import numpy as np
 import torch
 import torch.nn.functional as F
 

@@ -11,7 +11,6 @@ The raster gradients come from torch autograd of the differentiable binned
 forward below, which is independent of the closed-form adjoint in
 ``oracle.raster.render_backward``; the two are compared in the tests.
 """
-import numpy as np
 import torch
 
 from . import adan, binning, loss, model, optics, raster

@@ -12,8 +12,6 @@ FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13 on runtime plans, and the
 compile-time plans of the three standard workloads (transform lengths 1152; 2816 x 4480;
 4992 x 8320) at their real sizes.
 """
-import math
-
 import numpy as np
 import pytest
 import torch
