@@ -96,7 +96,7 @@ __device__ __forceinline__ void normals4(unsigned long long seed, uint32_t step,
 //   3. filters the ring vertically for map row q = y - 10 -> SSIM and the coefficient
 //      maps A, B, Cc of that row (never stored to HBM: staged in LDS for one step),
 //   4. applies the transposed horizontal filter to the map row and pushes the result
-//      into a second 11-row ring (in LDS, private to the thread's column),
+//      into a second 11-row register ring (private to the thread's column),
 //   5. for pixel row p = q, applies the transposed vertical filter from that ring and
 //      adds the L2 terms, the clamp mask and the target noise: dL/dI.  The pixel's
 //      intensity, target and mask were loaded during the previous step.
@@ -122,7 +122,6 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
     float* __restrict__ ssim_part, float* __restrict__ mse_part) {
   __shared__ float smo[2][5][NIN];   // staged input row: x, t, x^2, t^2, x t
   __shared__ float smap[2][3][NTH];
-  __shared__ float gring[WIN][3][NTH];
   __shared__ float red[2][NTH];
   const int i = threadIdx.x;
   const int x0 = blockIdx.x * OWN;
@@ -171,10 +170,11 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
   for (int k = 0; k < WIN; ++k)
 #pragma unroll
     for (int m = 0; m < 5; ++m) hm[k][m] = 0.f;
+  float gring[WIN][3];  // row-filtered coefficient-map ring of the thread's column
 #pragma unroll
   for (int k = 0; k < WIN; ++k)
 #pragma unroll
-    for (int m = 0; m < 3; ++m) gring[k][m][i] = 0.f;
+    for (int m = 0; m < 3; ++m) gring[k][m] = 0.f;
   float z4[4] = {0.f, 0.f, 0.f, 0.f};
   float ssum = 0.f, msum = 0.f;
   load_row(ys);
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
           g1 = fmaf(w, smap[buf][1][i - b], g1);
           g2 = fmaf(w, smap[buf][2][i - b], g2);
         }
-        gring[k][0][i] = g0; gring[k][1][i] = g1; gring[k][2][i] = g2;
+        gring[k][0] = g0; gring[k][1] = g1; gring[k][2] = g2;
       }
       // 5. pixel row p = q: transposed vertical filter over map rows p - a (slots k - a)
       const int pr = q;
@@ -278,9 +278,9 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
       for (int a = 0; a < WIN; ++a) {
         const int sl = (k - a + WIN) % WIN;
         const float w = P.win[a];
-        gA = fmaf(w, gring[sl][0][i], gA);
-        gB = fmaf(w, gring[sl][1][i], gB);
-        gC = fmaf(w, gring[sl][2][i], gC);
+        gA = fmaf(w, gring[sl][0], gA);
+        gB = fmaf(w, gring[sl][1], gB);
+        gC = fmaf(w, gring[sl][2], gC);
       }
       const float I = clamp01(cI);
       float nz = 0.f;
