@@ -35,6 +35,7 @@ constexpr float LAMBDA = 0.025f * 0.2f;
 
 struct LossParams {
   int C, D, H, W, Hv, Wv;
+  int own, rs;         // pixel columns per strip, pixel rows per segment (balanced)
   float win[WIN];
   float inv_n;         // 1 / (C H W)
   float ssim_coef;     // -LAMBDA / (C Hv Wv): dL/dS for every valid-map entry
@@ -101,21 +102,29 @@ __device__ __forceinline__ void normals4(unsigned long long seed, uint32_t step,
 //      adds the L2 terms, the clamp mask and the target noise: dL/dI.  The pixel's
 //      intensity, target and mask were loaded during the previous step.
 // The rings are indexed by the step modulo 11, which the 11-fold unrolled step loop
-// makes a compile-time choice.  Pixel rows [p0, p0 + RS) need map rows [p0 - 10,
-// p0 + RS) and input rows [p0 - 10, p0 + RS + 10): RS + 20 steps.
+// makes a compile-time choice.  Pixel rows [p0, p0 + rs) need map rows [p0 - 10,
+// p0 + rs) and input rows [p0 - 10, p0 + rs + 10): rs + 20 steps.  The host splits the
+// columns into equal strips of own <= OWN columns (threads i >= own + 10 of a narrower
+// strip only stage input columns) and the rows into equal segments, choosing the
+// segment count so that the workgroups fill whole passes of the GPU (loss_split).
 // ---------------------------------------------------------------------------------
 #ifndef CVQPG_LOSS_NTH
-#define CVQPG_LOSS_NTH 128
+#define CVQPG_LOSS_NTH 256
 #endif
 constexpr int NTH = CVQPG_LOSS_NTH;   // threads = map columns per strip
-constexpr int OWN = NTH - (WIN - 1);  // pixel columns per strip
+constexpr int OWN = NTH - (WIN - 1);  // most pixel columns per strip
 #ifndef CVQPG_LOSS_RS
 #define CVQPG_LOSS_RS 192
 #endif
-constexpr int RS = CVQPG_LOSS_RS;     // pixel rows per segment
+constexpr int RS = CVQPG_LOSS_RS;     // segment rows when the device cannot be queried
+constexpr int RS_MIN = 64;            // fewest rows per segment the split considers
+static_assert(RS % 4 == 0, "segments hold whole groups of four rows");
 constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 
-__global__ __launch_bounds__(NTH) void k_loss_stream(
+#ifndef CVQPG_LOSS_WPE
+#define CVQPG_LOSS_WPE 1
+#endif
+__global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
     const float* __restrict__ inten, const float* __restrict__ target,
     const float* __restrict__ mask, const float* __restrict__ noise,
     const int* __restrict__ step_counter, LossParams P, float* __restrict__ grad,
@@ -124,12 +133,12 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
   __shared__ float smap[2][3][NTH];
   __shared__ float red[2][NTH];
   const int i = threadIdx.x;
-  const int x0 = blockIdx.x * OWN;
-  const int p0 = blockIdx.y * RS;
+  const int x0 = blockIdx.x * P.own;
+  const int p0 = blockIdx.y * P.rs;
   const int dc = blockIdx.z;
   const int d = dc / P.C, c = dc % P.C;
   const int mc = x0 - (WIN - 1) + i;              // map column (and pixel column, i >= 10)
-  const bool own_col = i >= WIN - 1;
+  const bool own_col = i >= WIN - 1 && i < P.own + WIN - 1;
   const bool map_col = mc >= 0 && mc < P.Wv;
   const bool pix_col = own_col && mc < P.W;
   // plane offsets in 64 bits, offsets inside a plane (< 2^31 elements) in 32 bits
@@ -141,9 +150,9 @@ __global__ __launch_bounds__(NTH) void k_loss_stream(
   const int Wd = P.W;
   const uint32_t step = step_counter ? (uint32_t)step_counter[0] : 0u;
   const float ns = P.noise_scale[d];
-  const int p_end = min(p0 + RS, P.H);
+  const int p_end = min(p0 + P.rs, P.H);
   const int ys = p0 - (WIN - 1);                  // first input row
-  const int nsteps = RS + 2 * (WIN - 1);
+  const int nsteps = P.rs + 2 * (WIN - 1);
   const uint32_t hq = (uint32_t)((P.H + 3) / 4);
   // the input columns this thread stages: x0 - 10 + i and, for i < 10, x0 - 10 + i + NTH
   const int gx0 = mc, gx1 = mc + NTH;
@@ -351,15 +360,58 @@ struct LossWS {
   size_t bytes;
 };
 
-static dim3 loss_grid(int C, int D, int H, int W) {
-  return dim3(ceil_div(W, OWN), ceil_div(H, RS), D * C);
+// Workgroups of k_loss_stream the current device runs at once (CUs x workgroups per CU at
+// the kernel's register use), queried once per device; 0 if there is no device.
+static int loss_slots() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (!cached[dev]) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_loss_stream, NTH, 0) != hipSuccess)
+      return 0;
+    cached[dev] = cus * per;
+  }
+  return cached[dev];
+}
+
+static int round4(int v) { return (v + 3) & ~3; }
+
+// Strips of own <= OWN columns and segments of rs rows (a multiple of 4, the rows that share
+// one Philox draw), each as equal as the counts allow.  Every workgroup walks rs + 20 row
+// steps, so the kernel takes about (passes of the GPU) x (rs + 20) step times: the segment
+// count (rs >= RS_MIN) minimising that product is chosen, the fewest segments on a tie.
+// With 256 CUs holding 3 workgroups each, a 1920x1080 frame of 4 planes x 3 channels splits
+// into 8 strips x 8 segments of 136 rows: 768 workgroups, one full pass.
+static void loss_split(int slots, int DC, int H, int W, int& own, int& rs) {
+  const int strips = ceil_div(W, OWN);
+  own = ceil_div(W, strips);
+  if (slots <= 0) {
+    rs = round4(ceil_div(H, ceil_div(H, RS)));
+    return;
+  }
+  long best = -1;
+  for (int n = 1; n <= max(1, ceil_div(H, RS_MIN)); ++n) {
+    const int r = round4(ceil_div(H, n));
+    const long wgs = (long)strips * ceil_div(H, r) * DC;
+    const long cost = (long)ceil_div(wgs, (long)slots) * (r + 2 * (WIN - 1));
+    if (best < 0 || cost < best) {
+      best = cost;
+      rs = r;
+    }
+  }
+}
+
+// most workgroups any split launches (sizes the partial-sum workspace)
+static int loss_max_blocks(int C, int D, int H, int W) {
+  return ceil_div(W, OWN) * max(1, ceil_div(H, RS_MIN)) * D * C;
 }
 
 static LossWS carve_loss(void* base, int C, int D, int H, int W) {
   Carver cv(base);
   LossWS w;
-  const dim3 g = loss_grid(C, D, H, W);
-  const size_t nb = (size_t)g.x * g.y * g.z;
+  const size_t nb = (size_t)loss_max_blocks(C, D, H, W);
   w.ssim_part = cv.take<float>(nb);
   w.mse_part = cv.take<float>(nb);
   w.bytes = cv.bytes();
@@ -398,6 +450,7 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
   P.W = width;
   P.Hv = height - WIN + 1;
   P.Wv = width - WIN + 1;
+  loss_split(loss_slots(), planes * channels, height, width, P.own, P.rs);
   {
     double g[WIN], sum = 0.0;
     for (int i = 0; i < WIN; ++i) {
@@ -411,7 +464,7 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
   P.ssim_coef = (float)(-(double)LAMBDA / ((double)channels * P.Hv * P.Wv));
   for (int d = 0; d < 16; ++d) P.noise_scale[d] = (noise_scale && d < planes) ? noise_scale[d] : 0.f;
   P.seed = seed;
-  const dim3 g = loss_grid(channels, planes, height, width);
+  const dim3 g(ceil_div(width, P.own), ceil_div(height, P.rs), planes * channels);
   const int nb = (int)(g.x * g.y * g.z);
   hipLaunchKernelGGL(k_loss_stream, g, dim3(NTH), 0, s, intensity, target, mask, noise,
                      step_counter, P, grad_intensity, w.ssim_part, w.mse_part);
