@@ -518,17 +518,57 @@ __global__ __launch_bounds__(256) void k_render_fwd(const int2* __restrict__ ran
 }
 
 // ------------------------------------------------------------------------------
-// raster adjoint: one Gaussian per lane, pixels as wave-uniform broadcasts
+// raster adjoint: one Gaussian per lane, each lane walking its own rows of the tile
 // ------------------------------------------------------------------------------
 // partial record (16 floats): dcol[C], dph'[C], dop, K', di00, di01', di11, dmx', dmy'
-// One tile of one hologram; sgr is the workgroup's [C][256] pixel-gradient buffer,
-// s_keys a 512-entry queue of keys and s_cnt 4 counters (LDS).  A key whose Gaussian
-// stays below G = 2^-24 on every pixel of the tile adds less than 2^-24 of a unit term
-// to every adjoint sum (the reference evaluates it, down to its clamp at e^-50): such
-// keys get a zero partial record (so the per-Gaussian sums stay complete) and are not
-// evaluated.  The others are queued in key order and evaluated 256 at a time, one per
-// lane.  The first write to sgr is not preceded by a barrier here: a workgroup that calls
-// this for several tiles places one between the calls.
+constexpr int GROW = TILE + 1;   // LDS row pitch of the pixel gradients (float2): rows that
+                                 // lanes read at the same time fall in distinct bank pairs
+
+// LDS of one adjoint workgroup
+template <int C>
+struct BwdLds {
+  float2 grad[C][TILE * GROW];  // the tile's pixel gradients dL/dU, row pitch GROW
+  int keys[512];                // queued keys, in key order
+  int span[512];                // their rows: first significant row | row count << 8
+  int rkeys[256];               // one evaluation round, by row count (descending)
+  int rspan[256];
+  int cnt[4];
+  int hist[TILE + 1];
+};
+
+// Rows of the tile on which a Gaussian reaches 2^QSKIP: for each row, the maximum of the
+// exponent over the row's pixel span [x0, x1] (the concave quadratic's peak along the row,
+// clamped to the span).  The row maximum is concave in y, so the rows form one range;
+// returned as first | count << 8 (count 0: none).
+__device__ __forceinline__ int row_span(float mx, float my, float a00, float a01, float a11,
+                                        float x0, float x1, float y0, int ny) {
+  const float dxlo = mx - x1, dxhi = mx - x0, rx = -a01 / a00;
+  int first = TILE, last = -1;
+#pragma unroll
+  for (int yy = 0; yy < TILE; ++yy) {
+    const float dy = my - (y0 + (float)yy);
+    const float dx = fminf(fmaxf(rx * dy, dxlo), dxhi);
+    const float q = fmaf(dx, fmaf(a00, dx, 2.f * a01 * dy), a11 * dy * dy);
+    if (yy < ny && q >= QSKIP) {
+      first = min(first, yy);
+      last = yy;
+    }
+  }
+  return last < first ? 0 : first | ((last - first + 1) << 8);
+}
+
+// One tile of one hologram.  A key whose Gaussian stays below G = 2^-24 on every pixel of
+// the tile adds less than 2^-24 of a unit term to every adjoint sum (the reference
+// evaluates it, down to its clamp at e^-50): such keys get a zero partial record (so the
+// per-Gaussian sums stay complete) and are not evaluated.  The others are queued with the
+// range of rows on which they reach 2^-24 and evaluated 256 at a time, one per lane, each
+// lane summing over its own rows (rows outside the range add less than 2^-24 of a unit
+// term, like the dropped keys).  Every round is first sorted by row count, so that the 64
+// lanes of a wave walk similar numbers of rows: at 1080p this evaluates about three
+// quarters of the rows of the queued keys.  A key's record depends only on the key, so
+// the order in which lanes take keys does not change any result.  The first write to the
+// pixel gradients is not preceded by a barrier here: a workgroup that calls this for
+// several tiles places one between the calls.
 template <int C>
 __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict__ ranges,
                                                 const int32_t* __restrict__ point_list,
@@ -538,7 +578,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
                                                 const float* __restrict__ kvec,
                                                 const uint32_t* __restrict__ misc, int W, int H,
                                                 int gx, float4* __restrict__ partials,
-                                                float2 (*sgr)[256], int* s_keys, int* s_cnt) {
+                                                BwdLds<C>& L) {
   const int tx = tile % gx, ty = tile / gx;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -550,7 +590,8 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
     const bool inside = px < W && py < H;
 #pragma unroll
     for (int c = 0; c < C; ++c)
-      sgr[c][t] = inside ? grad[((size_t)c * H + py) * W + px] : make_float2(0.f, 0.f);
+      L.grad[c][(t >> 4) * GROW + (t & 15)] =
+          inside ? grad[((size_t)c * H + py) * W + px] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const int x0 = tx * TILE, y0 = ty * TILE;
@@ -559,7 +600,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
 #pragma unroll
   for (int c = 0; c < C; ++c) kc[c] = kvec[c];
 
-  auto process = [&](int g, bool valid) {
+  auto process = [&](int g, int span, bool valid) {
     const float4* src = packed + (size_t)g * (REC / 4);
     const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
     const float mx = q0.x, my = q0.y, a00 = q0.z, a01 = q0.w, a11 = q1.x;
@@ -589,11 +630,16 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
     for (int c = 0; c < C; ++c) dcol[c] = dph[c] = 0.f;
     float K = 0.f, di00 = 0.f, di01 = 0.f, di11 = 0.f, dmx = 0.f, dmy = 0.f;
     const float dx0 = mx - (float)x0;
-    for (int yy = 0; yy < ny; ++yy) {
+    const int r0 = span & 0xff, nr = span >> 8;
+    // the round is sorted by row count, so the wave's first lane walks the most rows
+    const int nmax = __builtin_amdgcn_readfirstlane(nr);
+    for (int k = 0; k < nmax; ++k) {
+      if (k >= nr) continue;
+      const int yy = r0 + k;
       const float dy = my - (float)(y0 + yy);
       const float bq = 2.0f * a01 * dy, cq = a11 * dy * dy;
       float S0 = 0.f, S1 = 0.f, S2 = 0.f;
-      const float2* gr0 = &sgr[0][yy * TILE];
+      const float2* gr0 = &L.grad[0][yy * GROW];
       auto pixel = [&](int xx) {
         const float dx = dx0 - (float)xx;
         const float q = fmaf(dx, fmaf(a00, dx, bq), cq);
@@ -601,7 +647,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
         float Sh = 0.f, Tb = 0.f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float2 gg = gr0[c * 256 + xx];
+          const float2 gg = gr0[c * TILE * GROW + xx];
           const float rev = fract_f(fmaf(w[c], q, u[c]));
           const float sn = sin_rev(rev), cs = cos_rev(rev);
           const float Pc = fmaf(gg.x, cs, gg.y * sn);
@@ -670,16 +716,17 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
 
   const float fx0 = (float)x0, fy0 = (float)y0;
   const float fx1 = (float)(x0 + nx - 1), fy1 = (float)(y0 + ny - 1);
-  int filled = 0;  // queued keys in s_keys[0, filled)
+  int filled = 0;  // queued keys in L.keys[0, filled)
   for (int base = rg.x; base < rg.y; base += 256) {
     const int i = base + t;
     bool keep = false;
-    int g = 0;
+    int g = 0, span = 0;
     if (i < rg.y) {
       g = point_list[i];
       const float4 q0 = packed[(size_t)g * (REC / 4)];
       const float a11 = packed[(size_t)g * (REC / 4) + 1].x;
       keep = tile_qmax(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, fy1) >= QSKIP;
+      if (keep) span = row_span(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, ny);
       if (!keep) {
         const int4 rc = rect[g];
         const int e = rc.x + (ty - rc.z) * rc.w + (tx - rc.y);
@@ -689,24 +736,55 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       }
     }
     const uint64_t bal = __ballot(keep);
-    if (lane == 0) s_cnt[wave] = __popcll(bal);
+    if (lane == 0) L.cnt[wave] = __popcll(bal);
     __syncthreads();
     int pos = filled + __popcll(bal & ((1ull << lane) - 1ull));
-    for (int w = 0; w < wave; ++w) pos += s_cnt[w];
-    if (keep) s_keys[pos] = g;
-    filled += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    __syncthreads();  // the queue is complete; s_cnt may be rewritten from here on
+    for (int w = 0; w < wave; ++w) pos += L.cnt[w];
+    if (keep) {
+      L.keys[pos] = g;
+      L.span[pos] = span;
+    }
+    filled += L.cnt[0] + L.cnt[1] + L.cnt[2] + L.cnt[3];
+    __syncthreads();  // the queue is complete; cnt may be rewritten from here on
     const bool last = base + 256 >= rg.y;
     while (filled >= 256 || (last && filled > 0)) {
       const int n = min(filled, 256);
-      if (wave * 64 < n) {
-        const int slot = wave * 64 + lane;
-        process(s_keys[slot < n ? slot : 0], slot < n);
+      // counting sort of the round's n keys by row count, most rows first; the order
+      // within one count is that of the LDS atomics and does not matter
+      if (t <= TILE) L.hist[t] = 0;
+      __syncthreads();
+      int rk = 0, sp = 0, key = 0;
+      if (t < n) {
+        key = L.keys[t];
+        sp = L.span[t];
+        rk = atomicAdd(&L.hist[TILE - (sp >> 8)], 1);
       }
-      __syncthreads();  // every lane has read its key
-      if (t < filled - 256) s_keys[t] = s_keys[256 + t];
+      __syncthreads();
+      if (t == 0) {
+        int acc = 0;
+        for (int b = 0; b <= TILE; ++b) {
+          const int c = L.hist[b];
+          L.hist[b] = acc;
+          acc += c;
+        }
+      }
+      __syncthreads();
+      if (t < n) {
+        const int at = L.hist[TILE - (sp >> 8)] + rk;
+        L.rkeys[at] = key;
+        L.rspan[at] = sp;
+      }
+      if (t < filled - 256) {
+        L.keys[t] = L.keys[256 + t];
+        L.span[t] = L.span[256 + t];
+      }
       filled -= n;
       __syncthreads();
+      if (wave * 64 < n) {
+        const int slot = wave * 64 + lane;
+        process(L.rkeys[slot < n ? slot : 0], slot < n ? L.rspan[slot] : 0, slot < n);
+      }
+      __syncthreads();  // every lane is done with rkeys / rspan
     }
   }
 }
@@ -723,11 +801,9 @@ __global__ __launch_bounds__(256, CVQPG_BWD_WPE) void k_render_bwd(const int2* _
                                                     const float* __restrict__ kvec,
                                                     const uint32_t* __restrict__ misc, int W,
                                                     int H, int gx, float4* __restrict__ partials) {
-  __shared__ float2 sgr[C][256];
-  __shared__ int s_keys[512];
-  __shared__ int s_cnt[4];
+  __shared__ BwdLds<C> lds;
   render_bwd_tile<C>(blockIdx.x, ranges, point_list, packed, rect, grad, kvec, misc, W, H, gx,
-                     partials, sgr, s_keys, s_cnt);
+                     partials, lds);
 }
 
 // ------------------------------------------------------------------------------
@@ -810,9 +886,7 @@ template <int C>
 __global__ __launch_bounds__(256, CVQPG_BWD_WPE) void k_render_bwd_batch(BatchHeader* __restrict__ hdr,
                                                           const BatchScene* __restrict__ sc,
                                                           int chunk) {
-  __shared__ float2 sgr[C][256];
-  __shared__ int s_keys[512];
-  __shared__ int s_cnt[4];
+  __shared__ BwdLds<C> lds;
   __shared__ int begin[MAX_BATCH];
   __shared__ int slot;
   const int ns = hdr->n_scenes, total = hdr->total_tiles;
@@ -823,11 +897,11 @@ __global__ __launch_bounds__(256, CVQPG_BWD_WPE) void k_render_bwd_batch(BatchHe
     const int last = min(first + chunk, total);
     int s = 0;
     for (int it = first; it < last; ++it) {
-      if (it > first) __syncthreads();  // the previous tile's waves are done with sgr
+      if (it > first) __syncthreads();  // the previous tile's waves are done with lds
       s = batch_scene_of(begin, ns, it, s);
       const BatchScene& S = sc[s];
       render_bwd_tile<C>(it - S.tile_begin, S.ranges, S.point_list, S.packed, S.rect, S.grad,
-                         S.k, S.misc, S.W, S.H, S.gx, S.partials, sgr, s_keys, s_cnt);
+                         S.k, S.misc, S.W, S.H, S.gx, S.partials, lds);
     }
   }
   batch_exit(hdr, 1);
