@@ -288,6 +288,8 @@ def main():
     print(f"[bench] building workload {args.workload}", file=sys.stderr, flush=True)
     wl = train.Workload(args.workload, device=dev, seed=0,
                         renderer="torch" if args.path == "torch" else "hip")
+    # the step's peak counts the workload's resident tensors, not the temporaries of its setup
+    torch.cuda.reset_peak_memory_stats(dev)
     pH, pW = synthetic.pad_size_for(W, H)
     f_fwd, f_bwd = raster_flops_per_pair(C)
     if args.path in ("autograd", "torch"):
@@ -341,6 +343,7 @@ def main():
                 "pairs": pairs, "flop_per_pair": f_bwd}
         if traffic is not None:
             roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9   # the step's own peak
     if args.calib_copy > 0:
         n = args.calib_copy * (1 << 20) // 4
         src = torch.ones(n, device=dev)
@@ -348,7 +351,6 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(f"[bench] calibration copy of {args.calib_copy} MiB done", file=sys.stderr, flush=True)
-    peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9   # the step's own peak
     warm = None
     if not args.no_warmstart and args.workload == WORKLOAD:
         try:

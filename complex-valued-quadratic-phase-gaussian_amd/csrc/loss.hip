@@ -117,7 +117,8 @@ constexpr int OWN = NTH - (WIN - 1);  // most pixel columns per strip
 #define CVQPG_LOSS_RS 192
 #endif
 constexpr int RS = CVQPG_LOSS_RS;     // segment rows when the device cannot be queried
-constexpr int RS_MIN = 64;            // fewest rows per segment the split considers
+constexpr int RS_MIN = 128;           // segments of at least ~RS_MIN rows: at most 20 / 128
+                                      // of the steps repeat halo rows
 static_assert(RS % 4 == 0, "segments hold whole groups of four rows");
 constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 
@@ -381,7 +382,8 @@ static int round4(int v) { return (v + 3) & ~3; }
 // Strips of own <= OWN columns and segments of rs rows (a multiple of 4, the rows that share
 // one Philox draw), each as equal as the counts allow.  Every workgroup walks rs + 20 row
 // steps, so the kernel takes about (passes of the GPU) x (rs + 20) step times: the segment
-// count (rs >= RS_MIN) minimising that product is chosen, the fewest segments on a tie.
+// count (at most ceil(H / RS_MIN)) minimising that product is chosen, the fewest segments
+// on a tie.
 // With 256 CUs holding 3 workgroups each, a 1920x1080 frame of 4 planes x 3 channels splits
 // into 8 strips x 8 segments of 136 rows: 768 workgroups, one full pass.
 static void loss_split(int slots, int DC, int H, int W, int& own, int& rs) {
