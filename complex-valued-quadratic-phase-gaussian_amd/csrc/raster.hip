@@ -423,8 +423,11 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
                                                 int* s_cnt) {
   const int tx = tile % gx, ty = tile / gx;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int px = tx * TILE + (t & 15);
-  const int py = ty * TILE + (t >> 4);
+  // each wave covers an 8x8 quarter of the tile: more compact than four rows of 16, so
+  // fewer (wave, Gaussian) pairs reach 2^-24 (44.5 % against 48.3 % of the tile lists'
+  // pairs at 1080p, counted on the CPU from the synthetic scene)
+  const int px = tx * TILE + ((t >> 6) & 1) * 8 + (t & 7);
+  const int py = ty * TILE + (t >> 7) * 8 + ((t >> 3) & 7);
   int2 rg = ranges[tile];
   if (misc[1]) rg = make_int2(0, 0);
   const float fpx = (float)px, fpy = (float)py;
