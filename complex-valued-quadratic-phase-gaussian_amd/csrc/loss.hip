@@ -125,12 +125,17 @@ constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 #ifndef CVQPG_LOSS_WPE
 #define CVQPG_LOSS_WPE 1
 #endif
+// TS: the target's window statistics (mu_t, E[t^2]) of every map entry come precomputed
+// (k_target_stats, bit-identical to filtering them here); only the three moments of the
+// intensity (x, x^2, x t) are then filtered per step.
+template <bool TS>
 __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
     const float* __restrict__ inten, const float* __restrict__ target,
     const float* __restrict__ mask, const float* __restrict__ noise,
-    const int* __restrict__ step_counter, LossParams P, float* __restrict__ grad,
-    float* __restrict__ ssim_part, float* __restrict__ mse_part) {
-  __shared__ float smo[2][5][NIN];   // staged input row: x, t, x^2, t^2, x t
+    const float2* __restrict__ tstats, const int* __restrict__ step_counter, LossParams P,
+    float* __restrict__ grad, float* __restrict__ ssim_part, float* __restrict__ mse_part) {
+  constexpr int NM = TS ? 3 : 5;     // filtered moments: x, x^2, x t (, t, t^2)
+  __shared__ float smo[2][NM][NIN];  // staged input row
   __shared__ float smap[2][3][NTH];
   __shared__ float red[2][NTH];
   const int i = threadIdx.x;
@@ -175,11 +180,17 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       if (nzp) qN = nzp[r];
     }
   };
-  float hm[WIN][5];   // moment ring: row-filtered x, t, x^2, t^2, xt of input rows
+  // target statistics of map row q at this thread's map column (TS)
+  const float2* tsp = TS ? tstats + (size_t)dc * P.Hv * P.Wv : nullptr;
+  float2 sN = make_float2(0.f, 0.f);
+  auto load_stats = [&](int q) {
+    if (TS && q >= 0 && q < P.Hv && map_col) sN = tsp[q * P.Wv + mc];
+  };
+  float hm[WIN][NM];   // moment ring: row-filtered moments of input rows
 #pragma unroll
   for (int k = 0; k < WIN; ++k)
 #pragma unroll
-    for (int m = 0; m < 5; ++m) hm[k][m] = 0.f;
+    for (int m = 0; m < NM; ++m) hm[k][m] = 0.f;
   float gring[WIN][3];  // row-filtered coefficient-map ring of the thread's column
 #pragma unroll
   for (int k = 0; k < WIN; ++k)
@@ -188,6 +199,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   float z4[4] = {0.f, 0.f, 0.f, 0.f};
   float ssum = 0.f, msum = 0.f;
   load_row(ys);
+  load_stats(ys - (WIN - 1));
 
   for (int base = 0; base < nsteps; base += WIN) {
 #pragma unroll
@@ -200,32 +212,34 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       //    pixel values of pixel row y - 9 (used next step)
       {
         const float a = clamp01(pa0), b = pb0;
-        smo[buf][0][i] = a; smo[buf][1][i] = b;
-        smo[buf][2][i] = a * a; smo[buf][3][i] = b * b; smo[buf][4][i] = a * b;
+        smo[buf][0][i] = a; smo[buf][1][i] = a * a; smo[buf][2][i] = a * b;
+        if (!TS) { smo[buf][NM - 2][i] = b; smo[buf][NM - 1][i] = b * b; }
         if (two) {
           const float a1 = clamp01(pa1), b1 = pb1;
-          smo[buf][0][i + NTH] = a1; smo[buf][1][i + NTH] = b1;
-          smo[buf][2][i + NTH] = a1 * a1; smo[buf][3][i + NTH] = b1 * b1;
-          smo[buf][4][i + NTH] = a1 * b1;
+          smo[buf][0][i + NTH] = a1; smo[buf][1][i + NTH] = a1 * a1;
+          smo[buf][2][i + NTH] = a1 * b1;
+          if (!TS) { smo[buf][NM - 2][i + NTH] = b1; smo[buf][NM - 1][i + NTH] = b1 * b1; }
         }
       }
       const float cI = qI, cT = qT, cM = qM, cN = qN;
+      const float2 cS = sN;
       load_row(y + 1);
       load_pix(y - (WIN - 1) + 1);
+      load_stats(y - (WIN - 1) + 1);
       __syncthreads();
       // 2. horizontal moments of map column mc (input columns mc .. mc + 10)
       {
-        float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+        float m[NM];
+#pragma unroll
+        for (int j = 0; j < NM; ++j) m[j] = 0.f;
 #pragma unroll
         for (int b = 0; b < WIN; ++b) {
           const float w = P.win[b];
-          m0 = fmaf(w, smo[buf][0][i + b], m0);
-          m1 = fmaf(w, smo[buf][1][i + b], m1);
-          m2 = fmaf(w, smo[buf][2][i + b], m2);
-          m3 = fmaf(w, smo[buf][3][i + b], m3);
-          m4 = fmaf(w, smo[buf][4][i + b], m4);
+#pragma unroll
+          for (int j = 0; j < NM; ++j) m[j] = fmaf(w, smo[buf][j][i + b], m[j]);
         }
-        hm[k][0] = m0; hm[k][1] = m1; hm[k][2] = m2; hm[k][3] = m3; hm[k][4] = m4;
+#pragma unroll
+        for (int j = 0; j < NM; ++j) hm[k][j] = m[j];
       }
       if (sidx < WIN - 1) continue;          // ring not yet full: no map row
       // 3. map row q = y - 10 from input rows q .. q + 10 (ring slots k + 1 + a mod 11)
@@ -238,10 +252,16 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
           const int sl = (k + 1 + a) % WIN;
           const float w = P.win[a];
           mu1 = fmaf(w, hm[sl][0], mu1);
-          mu2 = fmaf(w, hm[sl][1], mu2);
-          e11 = fmaf(w, hm[sl][2], e11);
-          e22 = fmaf(w, hm[sl][3], e22);
-          e12 = fmaf(w, hm[sl][4], e12);
+          e11 = fmaf(w, hm[sl][1], e11);
+          e12 = fmaf(w, hm[sl][2], e12);
+          if (!TS) {
+            mu2 = fmaf(w, hm[sl][NM - 2], mu2);
+            e22 = fmaf(w, hm[sl][NM - 1], e22);
+          }
+        }
+        if (TS) {
+          mu2 = cS.x;
+          e22 = cS.y;
         }
         const float s11 = e11 - mu1 * mu1, s22 = e22 - mu2 * mu2, s12 = e12 - mu1 * mu2;
         const float A1 = 2.f * mu1 * mu2 + SSIM_C1, A2 = 2.f * s12 + SSIM_C2;
@@ -325,6 +345,32 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   }
 }
 
+// Window statistics of the target for k_loss_stream<true>: (mu_t, E[t^2]) of every valid
+// map entry [D C][Hv][Wv], with the streaming kernel's FMA chains (row filter over columns
+// mc .. mc + 10, then column filter over rows q .. q + 10), so that the loss and its
+// gradient are bit-identical to filtering the target in the streaming kernel.
+// grid (ceil(Wv / 256), Hv, D C)
+__global__ __launch_bounds__(256) void k_target_stats(const float* __restrict__ target,
+                                                      LossParams P, float2* __restrict__ stats) {
+  const int mc = blockIdx.x * 256 + threadIdx.x, q = blockIdx.y, dc = blockIdx.z;
+  if (mc >= P.Wv) return;
+  const float* tp = target + (size_t)dc * P.H * P.W;
+  float mu = 0.f, e = 0.f;
+  for (int a = 0; a < WIN; ++a) {
+    const float* row = tp + (size_t)(q + a) * P.W + mc;
+    float h1 = 0.f, h2 = 0.f;
+#pragma unroll
+    for (int b = 0; b < WIN; ++b) {
+      const float t = row[b];
+      h1 = fmaf(P.win[b], t, h1);
+      h2 = fmaf(P.win[b], t * t, h2);
+    }
+    mu = fmaf(P.win[a], h1, mu);
+    e = fmaf(P.win[a], h2, e);
+  }
+  stats[((size_t)dc * P.Hv + q) * P.Wv + mc] = make_float2(mu, e);
+}
+
 // loss = sum_d [ mse_d / n + LAMBDA (1 - SSIM_d) ]; one workgroup, fixed summation order
 __global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ ssim_part,
                                                      int n_ssim, const float* __restrict__ mse_part,
@@ -361,8 +407,10 @@ struct LossWS {
   size_t bytes;
 };
 
-// Workgroups of k_loss_stream the current device runs at once (CUs x workgroups per CU at
-// the kernel's register use), queried once per device; 0 if there is no device.
+// Workgroups of k_loss_stream<TS> the current device runs at once (CUs x workgroups per
+// CU at the kernel's register and LDS use), queried once per device; 0 if there is no
+// device.
+template <bool TS>
 static int loss_slots() {
   static int cached[64] = {0};
   int dev = 0;
@@ -370,7 +418,8 @@ static int loss_slots() {
   if (!cached[dev]) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_loss_stream, NTH, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_loss_stream<TS>, NTH, 0) !=
+            hipSuccess)
       return 0;
     cached[dev] = cus * per;
   }
@@ -384,8 +433,10 @@ static int round4(int v) { return (v + 3) & ~3; }
 // steps, so the kernel takes about (passes of the GPU) x (rs + 20) step times: the segment
 // count (at most ceil(H / RS_MIN)) minimising that product is chosen, the fewest segments
 // on a tie.
-// With 256 CUs holding 3 workgroups each, a 1920x1080 frame of 4 planes x 3 channels splits
-// into 8 strips x 8 segments of 136 rows: 768 workgroups, one full pass.
+// With 256 CUs holding 3 workgroups each (the kernel filtering the target itself), a
+// 1920x1080 frame of 4 planes x 3 channels splits into 8 strips x 8 segments of 136 rows:
+// 768 workgroups, one full pass; with 4 per CU (precomputed target statistics), into
+// 8 strips x 9 segments of 120 rows.
 static void loss_split(int slots, int DC, int H, int W, int& own, int& rs) {
   const int strips = ceil_div(W, OWN);
   own = ceil_div(W, strips);
@@ -429,22 +480,8 @@ extern "C" size_t cvqpg_loss_workspace_size(int channels, int planes, int height
   return carve_loss(nullptr, channels, planes, height, width).bytes;
 }
 
-extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* target,
-                                           const float* mask, const float* noise,
-                                           const float* noise_scale, uint64_t seed,
-                                           const int32_t* step_counter, int channels,
-                                           int planes, int height, int width, void* ws,
-                                           size_t ws_bytes, float* grad_intensity, float* loss,
-                                           void* stream) {
-  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
-  CVQPG_CHECK_ARG(planes >= 1 && planes <= 16, "planes must be 1..16");
-  CVQPG_CHECK_ARG(height >= WIN && width >= WIN, "image smaller than the SSIM window");
-  CVQPG_CHECK_ARG(intensity && target && mask && grad_intensity && loss && ws,
-                  "NULL argument pointer");
-  const LossWS need = carve_loss(nullptr, channels, planes, height, width);
-  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
-  hipStream_t s = (hipStream_t)stream;
-  const LossWS w = carve_loss(ws, channels, planes, height, width);
+static LossParams loss_params(int channels, int planes, int height, int width,
+                              const float* noise_scale, uint64_t seed) {
   LossParams P;
   P.C = channels;
   P.D = planes;
@@ -452,7 +489,7 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
   P.W = width;
   P.Hv = height - WIN + 1;
   P.Wv = width - WIN + 1;
-  loss_split(loss_slots(), planes * channels, height, width, P.own, P.rs);
+  P.own = P.rs = 0;
   {
     double g[WIN], sum = 0.0;
     for (int i = 0; i < WIN; ++i) {
@@ -466,12 +503,73 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
   P.ssim_coef = (float)(-(double)LAMBDA / ((double)channels * P.Hv * P.Wv));
   for (int d = 0; d < 16; ++d) P.noise_scale[d] = (noise_scale && d < planes) ? noise_scale[d] : 0.f;
   P.seed = seed;
+  return P;
+}
+
+extern "C" size_t cvqpg_loss_target_stats_size(int channels, int planes, int height,
+                                               int width) {
+  if (height < WIN || width < WIN || channels < 1 || planes < 1) return 0;
+  return (size_t)planes * channels * (height - WIN + 1) * (width - WIN + 1) * sizeof(float2);
+}
+
+extern "C" int cvqpg_loss_target_stats(const float* target, int channels, int planes,
+                                       int height, int width, float* stats, size_t stats_bytes,
+                                       void* stream) {
+  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
+  CVQPG_CHECK_ARG(planes >= 1 && planes <= 16, "planes must be 1..16");
+  CVQPG_CHECK_ARG(height >= WIN && width >= WIN, "image smaller than the SSIM window");
+  CVQPG_CHECK_ARG(target && stats, "NULL argument pointer");
+  const size_t need = cvqpg_loss_target_stats_size(channels, planes, height, width);
+  CVQPG_CHECK_ARG(stats_bytes >= need, "statistics buffer too small: %zu < %zu", stats_bytes, need);
+  const LossParams P = loss_params(channels, planes, height, width, nullptr, 0);
+  hipLaunchKernelGGL(k_target_stats, dim3(ceil_div(P.Wv, 256), P.Hv, planes * channels),
+                     dim3(256), 0, (hipStream_t)stream, target, P,
+                     reinterpret_cast<float2*>(stats));
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_loss_forward_backward_stats(
+    const float* intensity, const float* target, const float* target_stats, const float* mask,
+    const float* noise, const float* noise_scale, uint64_t seed, const int32_t* step_counter,
+    int channels, int planes, int height, int width, void* ws, size_t ws_bytes,
+    float* grad_intensity, float* loss, void* stream) {
+  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
+  CVQPG_CHECK_ARG(planes >= 1 && planes <= 16, "planes must be 1..16");
+  CVQPG_CHECK_ARG(height >= WIN && width >= WIN, "image smaller than the SSIM window");
+  CVQPG_CHECK_ARG(intensity && target && mask && grad_intensity && loss && ws,
+                  "NULL argument pointer");
+  const LossWS need = carve_loss(nullptr, channels, planes, height, width);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const LossWS w = carve_loss(ws, channels, planes, height, width);
+  LossParams P = loss_params(channels, planes, height, width, noise_scale, seed);
+  const float2* ts = reinterpret_cast<const float2*>(target_stats);
+  loss_split(ts ? loss_slots<true>() : loss_slots<false>(), planes * channels, height, width,
+             P.own, P.rs);
   const dim3 g(ceil_div(width, P.own), ceil_div(height, P.rs), planes * channels);
   const int nb = (int)(g.x * g.y * g.z);
-  hipLaunchKernelGGL(k_loss_stream, g, dim3(NTH), 0, s, intensity, target, mask, noise,
-                     step_counter, P, grad_intensity, w.ssim_part, w.mse_part);
+  if (ts)
+    hipLaunchKernelGGL(k_loss_stream<true>, g, dim3(NTH), 0, s, intensity, target, mask, noise,
+                       ts, step_counter, P, grad_intensity, w.ssim_part, w.mse_part);
+  else
+    hipLaunchKernelGGL(k_loss_stream<false>, g, dim3(NTH), 0, s, intensity, target, mask,
+                       noise, ts, step_counter, P, grad_intensity, w.ssim_part, w.mse_part);
   hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(1024), 0, s, w.ssim_part, nb, w.mse_part, nb,
                      P, loss);
   CVQPG_LAUNCH_CHECK();
   return 0;
+}
+
+extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* target,
+                                           const float* mask, const float* noise,
+                                           const float* noise_scale, uint64_t seed,
+                                           const int32_t* step_counter, int channels,
+                                           int planes, int height, int width, void* ws,
+                                           size_t ws_bytes, float* grad_intensity, float* loss,
+                                           void* stream) {
+  return cvqpg_loss_forward_backward_stats(intensity, target, nullptr, mask, noise,
+                                           noise_scale, seed, step_counter, channels, planes,
+                                           height, width, ws, ws_bytes, grad_intensity, loss,
+                                           stream);
 }

@@ -7,7 +7,8 @@ on the current stream:
     binning       cvqpg_bin_count, cvqpg_bin_sort
     raster        cvqpg_render_forward_packed  -> field [C,H,W]
     optics        cvqpg_blasm_forward          -> plane fields + intensities [D,C,H,W]
-    loss          cvqpg_loss_forward_backward  -> dL/dI, loss
+    loss          cvqpg_loss_forward_backward_stats  -> dL/dI, loss (target statistics
+                  from cvqpg_loss_target_stats, once per fit)
     optics adj.   cvqpg_blasm_backward         -> dL/dfield
     raster adj.   cvqpg_render_backward_tiles + cvqpg_raster_backward_raw -> raw grads
     optimiser     cvqpg_adan_step              clip, Adan, schedule, iteration += 1
@@ -77,6 +78,12 @@ class FusedTrainer:
         self.loss_ws = torch.empty(int(nl), dtype=torch.uint8, device=dev)
         self.targets = wl.targets.contiguous().float()
         self.masks = wl.masks.contiguous().float()
+        # the targets are fixed for the fit: their SSIM window statistics are filtered once
+        nts = int(L.cvqpg_loss_target_stats_size(C, D, H, W))
+        self.target_stats = torch.empty(nts // 4, device=dev)
+        _lib.check(L.cvqpg_loss_target_stats(_lib.ptr(self.targets), C, D, H, W,
+                                             _lib.ptr(self.target_stats), nts,
+                                             _lib.stream_handle()))
         if inject_noise:
             rng = (self.targets.flatten(1).max(1).values - self.targets.flatten(1).min(1).values)
             self.noise_scale = [1e-3 * float(v) for v in rng.cpu()]
@@ -168,8 +175,9 @@ class FusedTrainer:
             ("blasm_fwd", lambda: _lib.check(L.cvqpg_blasm_forward(
                 fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
                 _lib.ptr(self.inten), S()))),
-            ("loss", lambda: _lib.check(L.cvqpg_loss_forward_backward(
-                _lib.ptr(self.inten), _lib.ptr(self.targets), _lib.ptr(self.masks), None, ns,
+            ("loss", lambda: _lib.check(L.cvqpg_loss_forward_backward_stats(
+                _lib.ptr(self.inten), _lib.ptr(self.targets), _lib.ptr(self.target_stats),
+                _lib.ptr(self.masks), None, ns,
                 self.seed, _lib.ptr(self.iteration), C, self.D, H, W, _lib.ptr(self.loss_ws),
                 self.loss_ws.numel(), _lib.ptr(self.grad_inten), _lib.ptr(self.loss), S()))),
             ("blasm_bwd", lambda: _lib.check(L.cvqpg_blasm_backward(

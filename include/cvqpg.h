@@ -232,6 +232,27 @@ int cvqpg_loss_forward_backward(const float *intensity, const float *target, con
                                 int height, int width, void *ws, size_t ws_bytes,
                                 float *grad_intensity, float *loss, void *stream);
 
+/* The same loss with the target's SSIM window statistics precomputed.  The reference
+ * recomputes them every call (GaussianLoss builds a new SSIM module per call,
+ * utils/data_utils.py:39-40) although the targets of a fit never change
+ * (set_targets, utils/propagator.py:361-415).  cvqpg_loss_target_stats writes
+ * (mu_T, E[T^2]) of every valid 11x11 window, [D,C,H-10,W-10] float2
+ * (cvqpg_loss_target_stats_size bytes), once per fit;
+ * cvqpg_loss_forward_backward_stats then filters only the intensity's moments per call
+ * and gives the same gradient, bit for bit, as cvqpg_loss_forward_backward, and the same
+ * loss up to the order in which its per-workgroup partial sums are added (target_stats
+ * NULL: identical to it). */
+size_t cvqpg_loss_target_stats_size(int channels, int planes, int height, int width);
+int cvqpg_loss_target_stats(const float *target, int channels, int planes, int height,
+                            int width, float *stats, size_t stats_bytes, void *stream);
+int cvqpg_loss_forward_backward_stats(const float *intensity, const float *target,
+                                      const float *target_stats, const float *mask,
+                                      const float *noise, const float *noise_scale,
+                                      uint64_t seed, const int32_t *step_counter, int channels,
+                                      int planes, int height, int width, void *ws,
+                                      size_t ws_bytes, float *grad_intensity, float *loss,
+                                      void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fused per-Gaussian prologue and its adjoint (the reference runs these in  */
 /* PyTorch: Gaussians2D.apply_activations / compute_2d_covariance_elements / */

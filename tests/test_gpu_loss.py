@@ -33,6 +33,30 @@ def run_gpu_loss(inten, target, mask, noise, scale, C, D, H, W, seed=0, step=Non
     return float(loss.item()), g.cpu().double()
 
 
+def run_gpu_loss_stats(inten, target, mask, noise, scale, C, D, H, W, seed=0, step=None):
+    """The loss through cvqpg_loss_target_stats + cvqpg_loss_forward_backward_stats."""
+    from cvqpg_amd import _lib
+    L = _lib.lib()
+    ws = torch.empty(int(L.cvqpg_loss_workspace_size(C, D, H, W)), dtype=torch.uint8, device=DEV)
+    nts = int(L.cvqpg_loss_target_stats_size(C, D, H, W))
+    assert nts == D * C * (H - 10) * (W - 10) * 8
+    stats = torch.empty(nts // 4, device=DEV)
+    g = torch.empty(D, C, H, W, device=DEV)
+    loss = torch.zeros(1, device=DEV)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float32).contiguous().to(DEV)
+    ti, tt, tm = t(inten), t(target), t(mask)
+    tn = t(noise) if noise is not None else None
+    _lib.check(L.cvqpg_loss_target_stats(_lib.ptr(tt), C, D, H, W, _lib.ptr(stats), nts,
+                                         _lib.stream_handle()))
+    _lib.check(L.cvqpg_loss_forward_backward_stats(
+        _lib.ptr(ti), _lib.ptr(tt), _lib.ptr(stats), _lib.ptr(tm), _lib.ptr(tn),
+        _lib.floats(scale) if scale is not None else None, seed,
+        _lib.ptr(step) if step is not None else None, C, D, H, W, _lib.ptr(ws), ws.numel(),
+        _lib.ptr(g), _lib.ptr(loss), _lib.stream_handle()))
+    torch.cuda.synchronize()
+    return float(loss.item()), g.cpu().double(), stats.view(D, C, H - 10, W - 10, 2).cpu().double()
+
+
 def oracle_loss(inten, target, mask, noise, scale):
     I = torch.as_tensor(inten, dtype=torch.float64).requires_grad_()
     T = torch.as_tensor(target, dtype=torch.float64)
@@ -88,3 +112,31 @@ def test_in_kernel_noise_statistics(pkg):
         print(f"noise mean {z.mean():.4f} std {z.std():.4f}", flush=True)
         assert abs(z.mean()) < 0.03 and abs(z.std() - 1.0) < 0.03
     assert np.corrcoef(zs[0], zs[1])[0, 1] < 0.05
+
+
+@pytest.mark.parametrize("C,D,H,W", [(3, 4, 40, 56), (1, 2, 75, 139), (3, 2, 1080, 1920)])
+def test_target_statistics_path(pkg, C, D, H, W):
+    """Precomputed target window statistics: (mu_T, E[T^2]) within 1e-6 of the FP64
+    separable filter of the oracle, and the loss kernel that reads them gives the same
+    gradient bit for bit as the one that filters the target itself (with a noise tensor
+    and with in-kernel Philox noise); the loss agrees up to the order of its partial sums."""
+    rng = np.random.default_rng(H * 7 + W)
+    inten = rng.uniform(-0.1, 1.2, (D, C, H, W)).astype(np.float32)
+    target = rng.uniform(0, 1, (D, C, H, W)).astype(np.float32)
+    mask = (rng.uniform(size=(D, C, H, W)) > 0.5).astype(np.float32)
+    noise = rng.normal(size=(D, C, H, W)).astype(np.float32)
+    scale = [1e-3 * float(target[d].max() - target[d].min()) for d in range(D)]
+    step = torch.full((1,), 3, dtype=torch.int32, device=DEV)
+    for nz in (noise, None):
+        lg, gg = run_gpu_loss(inten, target, mask, nz, scale, C, D, H, W, seed=11, step=step)
+        ls, gs, st = run_gpu_loss_stats(inten, target, mask, nz, scale, C, D, H, W, seed=11,
+                                        step=step)
+        assert torch.equal(gg, gs)
+        assert abs(lg - ls) <= 1e-6 * abs(lg)
+    T = torch.as_tensor(target, dtype=torch.float64).view(D * C, 1, H, W)
+    win = oloss.gauss_1d()
+    mu = oloss.gfilter(T, win).view(D, C, H - 10, W - 10)
+    e2 = oloss.gfilter(T * T, win).view(D, C, H - 10, W - 10)
+    err = max(float((st[..., 0] - mu).abs().max()), float((st[..., 1] - e2).abs().max()))
+    print(f"case C={C} D={D} {H}x{W}: statistics max abs error {err:.1e}", flush=True)
+    assert err <= 1e-6

@@ -58,6 +58,13 @@ def test_workspace_size_and_argument_errors(pkg):
     assert b"1 or 3 channels" in L.cvqpg_last_error()
     rc = L.cvqpg_bin_count(None, None, 10, 64, 64, None, 0, 0, None, None)
     assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+    # target window statistics: (D C) x (H - 10) x (W - 10) float2, checked before any launch
+    assert L.cvqpg_loss_target_stats_size(3, 4, 40, 56) == 4 * 3 * 30 * 46 * 8
+    assert L.cvqpg_loss_target_stats_size(3, 4, 10, 56) == 0
+    rc = L.cvqpg_loss_target_stats(None, 3, 4, 40, 56, None, 0, None)
+    assert rc == 1 and b"NULL" in L.cvqpg_last_error()
+    rc = L.cvqpg_loss_target_stats(1, 3, 4, 40, 56, 1, 100, None)
+    assert rc == 1 and b"too small" in L.cvqpg_last_error()
 
 
 def test_batch_table_layout_and_errors(pkg):
