@@ -102,14 +102,21 @@ static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int 
 }
 
 // two-level twiddle table of length n: W_n^{64 h} for h < nh, then W_n^l for l < 64
-__global__ void k_twiddles(float2* __restrict__ tw, int n) {
+__device__ __forceinline__ void twiddle_entry(float2* __restrict__ tw, int n, int i) {
   const int nh = (n + fft::TW_LO - 1) / fft::TW_LO;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nh + fft::TW_LO) return;
   const long m = i < nh ? (long)i * fft::TW_LO : (long)(i - nh);
   double s, c;
   sincospi(2.0 * (double)m / (double)n, &s, &c);
   tw[i] = make_float2((float)c, (float)-s);
+}
+
+// the tables of both transform lengths (rows nv, columns nu) in one launch; blockIdx.y
+// picks the table
+__global__ void k_twiddles(float2* __restrict__ tw_r, int nu, float2* __restrict__ tw_c, int nv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y == 0) twiddle_entry(tw_r, nu, i);
+  else twiddle_entry(tw_c, nv, i);
 }
 
 // copy the plan's twiddle table into LDS; the caller's next barrier publishes it
@@ -1169,8 +1176,9 @@ static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc
   CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "FFT length %d has a prime factor above 17", o.nv);
   pr.tw = w.tw_r;
   pc.tw = w.tw_c;
-  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(fft::twiddle_entries(o.nu), 256)), dim3(256), 0, s, w.tw_r, o.nu);
-  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(fft::twiddle_entries(o.nv), 256)), dim3(256), 0, s, w.tw_c, o.nv);
+  const int ne = std::max(fft::twiddle_entries(o.nu), fft::twiddle_entries(o.nv));
+  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, o.nu, w.tw_c,
+                     o.nv);
   return 0;
 }
 

@@ -334,9 +334,11 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
   }
 }
 
-__global__ void k_fill_i32(int32_t* __restrict__ p, int32_t v, int64_t n) {
+__global__ void k_clear_ranges(int32_t* __restrict__ ranges, int64_t n,
+                               int32_t* __restrict__ point_list) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  if (i < n) ranges[i] = 0;
+  if (i == 0) point_list[0] = 0;
 }
 
 __global__ void k_copy_u32(const uint32_t* __restrict__ src, int32_t* __restrict__ dst,
@@ -1240,9 +1242,9 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   const int T = gx * gy;
   const int64_t cap = capacity > 0 ? capacity : 1;
-  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(2 * (int64_t)T, 256)), dim3(256), 0, s, ranges, 0,
-                     2 * (int64_t)T);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, point_list, 0, (int64_t)1);
+  // untouched tiles keep (0, 0); point_list[0] = 0 (the reference's list when no key exists)
+  hipLaunchKernelGGL(k_clear_ranges, dim3(ceil_div(2 * (int64_t)T, 256)), dim3(256), 0, s, ranges,
+                     2 * (int64_t)T, point_list);
   if (n > 0)
     hipLaunchKernelGGL(k_duplicate, dim3(ceil_div(n, 256)), dim3(256), 0, s, w.counts, w.offs,
                        w.rect, w.misc, n, gx, w.keys_a, w.vals_a);
@@ -1252,6 +1254,8 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   uint32_t *kin = w.keys_a, *vin = w.vals_a, *kout = w.keys_b, *vout = w.vals_b;
   for (int shift = 0; shift < bits; shift += 8) {
     const int nbits = bits - shift < 8 ? bits - shift : 8;
+    // the last pass scatters the Gaussian ids straight into point_list
+    if (shift + 8 >= bits) vout = reinterpret_cast<uint32_t*>(point_list);
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(SORT_THREADS), 0, s, kin, w.misc, shift,
                        (1u << nbits) - 1u, nb, w.hist);
     CVQPG_LAUNCH_CHECK();
@@ -1268,8 +1272,9 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
     vin = vout;
     vout = tv;
   }
-  hipLaunchKernelGGL(k_copy_u32, dim3(ceil_div(cap, 256)), dim3(256), 0, s, vin, point_list,
-                     w.misc);
+  if (bits == 0)  // a single tile: nothing to sort
+    hipLaunchKernelGGL(k_copy_u32, dim3(ceil_div(cap, 256)), dim3(256), 0, s, vin, point_list,
+                       w.misc);
   hipLaunchKernelGGL(k_ranges, dim3(ceil_div(cap, 256)), dim3(256), 0, s, kin, w.misc,
                      reinterpret_cast<int2*>(ranges));
   CVQPG_LAUNCH_CHECK();
