@@ -117,8 +117,11 @@ constexpr int OWN = NTH - (WIN - 1);  // most pixel columns per strip
 #define CVQPG_LOSS_RS 192
 #endif
 constexpr int RS = CVQPG_LOSS_RS;     // segment rows when the device cannot be queried
-constexpr int RS_MIN = 128;           // segments of at least ~RS_MIN rows: at most 20 / 128
-                                      // of the steps repeat halo rows
+#ifndef CVQPG_LOSS_RS_MIN
+#define CVQPG_LOSS_RS_MIN 100
+#endif
+constexpr int RS_MIN = CVQPG_LOSS_RS_MIN;  // segments of at least ~RS_MIN rows: at most
+                                           // 20 / RS_MIN of the steps repeat halo rows
 static_assert(RS % 4 == 0, "segments hold whole groups of four rows");
 constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 
@@ -436,7 +439,7 @@ static int round4(int v) { return (v + 3) & ~3; }
 // With 256 CUs holding 3 workgroups each (the kernel filtering the target itself), a
 // 1920x1080 frame of 4 planes x 3 channels splits into 8 strips x 8 segments of 136 rows:
 // 768 workgroups, one full pass; with 4 per CU (precomputed target statistics), into
-// 8 strips x 9 segments of 120 rows.
+// 8 strips x 10 segments of 108 rows: 960 workgroups, one pass.
 static void loss_split(int slots, int DC, int H, int W, int& own, int& rs) {
   const int strips = ceil_div(W, OWN);
   own = ceil_div(W, strips);
