@@ -53,24 +53,9 @@ __global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restr
     s_coef = (float)(coef < 1.0 ? coef : 1.0);
   }
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + t;
   const int64_t total = A.off[7];
-  if (i >= total) return;
-  int seg = 0;
-#pragma unroll
-  for (int k = 1; k < 7; ++k) seg += i >= A.off[k] ? 1 : 0;
   const int itr = iteration[0];
   const int step = itr + 1;  // every group's step counter advances on every call
-  bool first = step == 1;
-  if (seg == A.curv_seg) {
-    if (!A.curv_enabled || itr < A.curv_warmup) return;  // no gradient: untouched
-    first = first || itr == A.curv_warmup;               // state created at unfreeze
-  }
-  float lr = A.lr[seg];
-  if (seg == 0) {
-    const double T = A.num_itrs > 1 ? A.num_itrs : 1;
-    lr = (float)(A.eta_min + (A.lr0 - A.eta_min) * (1.0 + cos(3.141592653589793 * itr / T)) / 2.0);
-  }
   const float bc1 = 1.0f - powf(A.b1, (float)step);
   const float bc2 = 1.0f - powf(A.b2, (float)step);
   const float bc3 = 1.0f - powf(A.b3, (float)step);
@@ -78,23 +63,40 @@ __global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restr
   float* v = state + total;
   float* dd = state + 2 * total;
   float* npg = state + 3 * total;
-  const float g = grads[i] * s_coef;
-  const float prev = first ? -g : npg[i];
-  const float diff = prev + g;
-  const float mi = m[i] * A.b1 + (1.0f - A.b1) * g;
-  const float di = dd[i] * A.b2 + (1.0f - A.b2) * diff;
-  const float u = diff * A.b2 + g;
-  const float vi = v[i] * A.b3 + (1.0f - A.b3) * u * u;
-  const float denom = sqrtf(vi) / sqrtf(bc3) + A.eps;
-  float* p = A.param[seg] + (i - A.off[seg]);
-  float pv = *p;
-  pv -= (lr / bc1) * mi / denom;
-  pv -= (lr * A.b2 / bc2) * di / denom;
-  *p = pv;
-  m[i] = mi;
-  v[i] = vi;
-  dd[i] = di;
-  npg[i] = -g;
+  // cosine schedule of the means group (MeansCosineScheduler, train_2d_gaussian_rgb.py:58-70)
+  const double T = A.num_itrs > 1 ? A.num_itrs : 1;
+  const float lr_means =
+      (float)(A.eta_min + (A.lr0 - A.eta_min) * (1.0 + cos(3.141592653589793 * itr / T)) / 2.0);
+  // grid-stride: every workgroup repeats the (small) norm reduction above once
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int seg = 0;
+#pragma unroll
+    for (int k = 1; k < 7; ++k) seg += i >= A.off[k] ? 1 : 0;
+    bool first = step == 1;
+    if (seg == A.curv_seg) {
+      if (!A.curv_enabled || itr < A.curv_warmup) continue;  // no gradient: untouched
+      first = first || itr == A.curv_warmup;                 // state created at unfreeze
+    }
+    const float lr = seg == 0 ? lr_means : A.lr[seg];
+    const float g = grads[i] * s_coef;
+    const float prev = first ? -g : npg[i];
+    const float diff = prev + g;
+    const float mi = m[i] * A.b1 + (1.0f - A.b1) * g;
+    const float di = dd[i] * A.b2 + (1.0f - A.b2) * diff;
+    const float u = diff * A.b2 + g;
+    const float vi = v[i] * A.b3 + (1.0f - A.b3) * u * u;
+    const float denom = sqrtf(vi) / sqrtf(bc3) + A.eps;
+    float* p = A.param[seg] + (i - A.off[seg]);
+    float pv = *p;
+    pv -= (lr / bc1) * mi / denom;
+    pv -= (lr * A.b2 / bc2) * di / denom;
+    *p = pv;
+    m[i] = mi;
+    v[i] = vi;
+    dd[i] = di;
+    npg[i] = -g;
+  }
 }
 
 __global__ void k_advance(int32_t* iteration) { iteration[0] += 1; }
@@ -133,8 +135,8 @@ extern "C" int cvqpg_adan_step(float* const* params, const int64_t* sizes, const
   hipStream_t s = (hipStream_t)stream;
   const int64_t total = A.off[7];
   if (total > 0)
-    hipLaunchKernelGGL(k_adan, dim3(ceil_div(total, 256)), dim3(256), 0, s, A, grads, state,
-                       sumsq_partials, n_partials, iteration);
+    hipLaunchKernelGGL(k_adan, dim3(std::min<int64_t>(ceil_div(total, 256), 1024)), dim3(256), 0,
+                       s, A, grads, state, sumsq_partials, n_partials, iteration);
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, iteration);
   CVQPG_LAUNCH_CHECK();
   return 0;
