@@ -241,26 +241,48 @@ __global__ void k_count(const float2* __restrict__ means, const int32_t* __restr
   rect[g] = make_int4(0, lx, ly, c ? hx - lx : 0);
 }
 
-__global__ void k_duplicate(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offs,
-                            int4* __restrict__ rect, const uint32_t* __restrict__ misc, int n,
-                            int gx, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n || misc[1]) return;
-  const uint32_t c = counts[g];
-  uint32_t off = offs[g];
-  int4 rc = rect[g];
-  rc.x = (int)off;
-  rect[g] = rc;
-  if (!c) return;
-  const int w = rc.w;
-  const int h = (int)c / w;
-  for (int y = 0; y < h; ++y) {
-    const uint32_t row = (uint32_t)(rc.z + y) * (uint32_t)gx + (uint32_t)rc.y;
-    for (int x = 0; x < w; ++x) {
-      keys[off] = row + (uint32_t)x;
-      vals[off] = (uint32_t)g;
-      ++off;
+// The keys of one workgroup's 256 Gaussians fill one contiguous range of the key array
+// (their exclusive offsets are consecutive), so the workgroup emits them together:
+// thread t writes entries t, t + 256, ... of the range (coalesced), finding the entry's
+// Gaussian by a binary search over the workgroup's offsets in LDS.  Each Gaussian's keys
+// run over its rectangle row by row, as in the reference.  grid ceil(n / 256)
+__global__ __launch_bounds__(256) void k_duplicate(const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ offs,
+                                                   int4* __restrict__ rect,
+                                                   const uint32_t* __restrict__ misc, int n,
+                                                   int gx, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  __shared__ uint32_t s_off[257];  // offsets relative to the range start; [256] = its end
+  __shared__ int4 s_rc[256];
+  if (misc[1]) return;  // uniform: the keys overflowed the capacity
+  const int t = threadIdx.x, g0 = blockIdx.x * 256, g = g0 + t;
+  const int last = min(g0 + 255, n - 1);
+  const uint32_t base = offs[g0];
+  const uint32_t end = offs[last] + counts[last] - base;
+  int4 rc = make_int4(0, 0, 0, 1);
+  uint32_t rel = end;
+  if (g < n) {
+    rel = offs[g] - base;
+    rc = rect[g];
+    rc.x = (int)(base + rel);
+    rect[g] = rc;
+  }
+  s_off[t] = rel;
+  s_rc[t] = rc;
+  if (t == 0) s_off[256] = end;
+  __syncthreads();
+  for (uint32_t i = t; i < end; i += 256) {
+    int lo = 0, hi = 255;  // the last j with s_off[j] <= i has s_off[j + 1] > i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_off[mid] <= i) lo = mid;
+      else hi = mid - 1;
     }
+    const int4 r = s_rc[lo];
+    const uint32_t local = i - s_off[lo];
+    const uint32_t y = local / (uint32_t)r.w, x = local - y * (uint32_t)r.w;
+    keys[base + i] = (uint32_t)(r.z + (int)y) * (uint32_t)gx + (uint32_t)r.y + x;
+    vals[base + i] = (uint32_t)(g0 + lo);
   }
 }
 
