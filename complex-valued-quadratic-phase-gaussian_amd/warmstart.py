@@ -393,23 +393,43 @@ def encode_targets(params, W, H):
     return target.view(Ty, Tx, OUT_DIM), weight.view(Ty, Tx, OUT_DIM)
 
 
+class _StraightClamp(torch.autograd.Function):
+    """Clamp in the forward, identity in the backward."""
+
+    @staticmethod
+    def forward(ctx, x, lo, hi):
+        return x.clamp(lo, hi)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g, None, None
+
+
+def _clamp(x, lo=None, hi=None):
+    """Clamp whose gradient passes through unchanged (straight-through): a slot output
+    pushed past a bound still learns, instead of going dead (a colour below 0 renders
+    nothing and would never receive a gradient again)."""
+    return _StraightClamp.apply(x, lo, hi)
+
+
 def decode(out, W, H, channels=3, ref=None, min_norm=0.5):
     """Slot encodings [Ty, Tx, 144] -> (seven raw parameter tensors of Ty*Tx*8 Gaussians,
     presence mask [Ty*Tx*8]).  With ``ref`` (a [144] encoding), a slot's phase whose
     predicted (cos, sin) vector is shorter than ``min_norm`` (an uncertain phase) takes
-    the reference slot's phase instead."""
+    the reference slot's phase instead.  The clamps pass gradients straight through
+    (``_clamp``); the decoded values are the clamped ones."""
     Ty, Tx = out.shape[:2]
     o = out.reshape(Ty * Tx * SLOTS, SLOT_DIM).float()
     t = torch.arange(Ty * Tx, device=out.device).repeat_interleave(SLOTS)
-    mx = 16 * (t % Tx).float() + (o[:, 0].clamp(-1, 1) + 1) * 8
-    my = 16 * (t // Tx).float() + (o[:, 1].clamp(-1, 1) + 1) * 8
-    mx = mx.clamp(0.01, W - 0.01)
-    my = my.clamp(0.01, H - 0.01)
-    means = torch.stack([torch.atanh((2 * mx / W - 1).clamp(-0.9999, 0.9999)),
-                         torch.atanh((2 * my / H - 1).clamp(-0.9999, 0.9999))], 1)
-    scales = o[:, 2:4].clamp(-4.0, 3.0)
+    mx = 16 * (t % Tx).float() + (_clamp(o[:, 0], -1, 1) + 1) * 8
+    my = 16 * (t // Tx).float() + (_clamp(o[:, 1], -1, 1) + 1) * 8
+    mx = _clamp(mx, 0.01, W - 0.01)
+    my = _clamp(my, 0.01, H - 0.01)
+    means = torch.stack([torch.atanh(_clamp(2 * mx / W - 1, -0.9999, 0.9999)),
+                         torch.atanh(_clamp(2 * my / H - 1, -0.9999, 0.9999))], 1)
+    scales = _clamp(o[:, 2:4], -4.0, 3.0)
     rot = 0.5 * torch.atan2(o[:, 5], o[:, 4])
-    colours = torch.stack([o[:, 6 + c] for c in range(channels)], 1).clamp(min=0.0)
+    colours = _clamp(torch.stack([o[:, 6 + c] for c in range(channels)], 1), 0.0)
     phase = torch.stack([torch.atan2(o[:, 10 + 2 * c], o[:, 9 + 2 * c]) for c in range(channels)], 1)
     if ref is not None:
         r = ref.reshape(SLOTS, SLOT_DIM).float().repeat(Ty * Tx, 1)
@@ -418,8 +438,8 @@ def decode(out, W, H, channels=3, ref=None, min_norm=0.5):
             rp = torch.atan2(r[:, 10 + 2 * c], r[:, 9 + 2 * c])
             phase[:, c] = torch.where(norm < min_norm, rp, phase[:, c])
     phase = torch.remainder(phase, 2 * math.pi)
-    curv = torch.atanh(o[:, 15].clamp(-0.99, 0.99))
-    opac = torch.logit(o[:, 16].clamp(0.01, 0.99))
+    curv = torch.atanh(_clamp(o[:, 15], -0.99, 0.99))
+    opac = torch.logit(_clamp(o[:, 16], 0.01, 0.99))
     present = o[:, 17] >= 0.5
     return [means.contiguous(), scales.contiguous(), rot.contiguous(), colours.contiguous(),
             phase.contiguous(), opac.contiguous(), curv.contiguous()], present
@@ -463,15 +483,17 @@ def warm_params(net, wl, ref=None):
     return params_from_slots(net.forward(network_input(wl)[None])[0], wl, ref=ref)
 
 
-def finetune(net, wls, steps, lr=3e-4, log=None):
+def finetune(net, wls, steps, lr=3e-4, log=None, checkpoint=None, every=250):
     """Render-loss fine-tuning: for one scene per step, the network's slots are decoded
     into the scene's first Ty*Tx*8 Gaussians (the rest keep their random start), the
     fused step runs without its optimiser (cvqpg kernels: raster, BLASM, loss, adjoints),
     the raw-parameter gradients flow back through the decode (torch autograd on the
     decode's elementwise maps) into the network's output, and the network takes one Adam
     step on its own kernels.  The objective is the fit's own loss at its first step, so the
-    network learns starts that already render close to the targets.  Returns the loss
-    history (every 50 steps)."""
+    network learns starts that already render close to the targets.  ``checkpoint(k)``,
+    when given, is called after every ``every`` steps and after the last one (the caller
+    validates and keeps the best network state).  Returns the loss history (every 50
+    steps)."""
     from .fused import FusedTrainer
     from .model import PARAM_NAMES
     trs = [FusedTrainer(wl, num_itrs=1000, warmup_iters=0, inject_noise=False) for wl in wls]
@@ -496,9 +518,13 @@ def finetune(net, wls, steps, lr=3e-4, log=None):
         A = net.activations(1, *out.shape[1:3])
         A["grad"]["out"].copy_(o.grad[None])
         net.backward(A)
-        net.adam(lr * 0.5 * (1 + math.cos(math.pi * k / steps)))
+        # cosine schedule after a 50-step linear warm-up (Adam's first bias-corrected
+        # steps move every weight by about lr)
+        net.adam(lr * min(1.0, (k + 1) / 50) * 0.5 * (1 + math.cos(math.pi * k / steps)))
         if k % 50 == 0 or k == steps - 1:
             hist.append((k, tr.loss.clone()))
+        if checkpoint is not None and ((k + 1) % every == 0 or k == steps - 1):
+            checkpoint(k + 1)
     out = [(k, float(v)) for k, v in hist]
     if log:
         log("render fine-tune loss " + " ".join(f"{k}:{v:.4f}" for k, v in out[::max(1, len(out) // 8)]))
@@ -722,9 +748,10 @@ def experiment(device="cuda", train_size=(1920, 1080), n_train=16, fit_iters=600
     2. supervised training on the pairs' slot encodings, from a head biased to their mean;
     3. render-loss fine-tuning (``finetune``) on ``ft_scenes`` scenes of ``ft_size``
        (windows cut from ``ft_source`` frames, or whole frames when it is None);
-    4. validation: the first-step loss on ``val_scenes`` further scenes chooses between the
-       fine-tuned network (every slot) and the supervised one (present slots, gated
-       phases); the plain start stays the fallback and the judge;
+    4. validation: the first-step loss on ``val_scenes`` further scenes picks the best
+       fine-tuning checkpoint (every 250 steps) and then chooses between it (every slot)
+       and the supervised network (present slots, gated phases); the plain start stays
+       the fallback and the judge;
     5. time-to-PSNR on held-out scenes of the chosen warm start against the plain start,
        and of the image-independent start from the mean slot encoding, which isolates
        what the network's conditioning on the image adds.
@@ -759,14 +786,28 @@ def experiment(device="cuda", train_size=(1920, 1080), n_train=16, fit_iters=600
         sps, sds = crop_specs(ft_size, ft_source, n, seed + salt)
         return [make_workload(sp, s, dev) for sp, s in zip(sps, sds)]
 
-    t2 = _time.perf_counter()
-    ft_hist = finetune(net, scenes(ft_scenes, 7), ft_steps, ft_lr, log=log) if ft_steps else []
-    torch.cuda.synchronize(dev)
-    t_ft = _time.perf_counter() - t2
     val = scenes(val_scenes, 11)
     tuned = lambda wl: all_slot_params(net.forward(network_input(wl)[None])[0], wl)
     gated = lambda wl: warm_params(sup, wl, ref=tm)
-    loss_tuned = start_loss(tuned, val) if ft_steps else float("inf")
+    # the render loss is not monotone over fine-tuning (a step on one scene can undo
+    # others): the network is validated every 250 steps and the best state is kept
+    best = dict(loss=float("inf"), state=None, step=None)
+
+    def checkpoint(k):
+        lv = start_loss(tuned, val)
+        if lv < best["loss"]:
+            best.update(loss=lv, state=net.state_dict(), step=k)
+        if log:
+            log(f"fine-tune step {k}: validation first-step loss {lv:.4f}")
+
+    t2 = _time.perf_counter()
+    ft_hist = (finetune(net, scenes(ft_scenes, 7), ft_steps, ft_lr, log=log,
+                        checkpoint=checkpoint) if ft_steps else [])
+    if best["state"] is not None:
+        net.load_state_dict(best["state"])
+    torch.cuda.synchronize(dev)
+    t_ft = _time.perf_counter() - t2
+    loss_tuned = best["loss"]
     loss_gated = start_loss(gated, val)
     choice = "fine-tuned" if loss_tuned <= loss_gated else "supervised"
     if log:
@@ -781,7 +822,8 @@ def experiment(device="cuda", train_size=(1920, 1080), n_train=16, fit_iters=600
                                ft_scenes=ft_scenes, ft_steps=ft_steps,
                                ft_loss_first=ft_hist[0][1] if ft_hist else None,
                                ft_loss_last=ft_hist[-1][1] if ft_hist else None,
-                               val_loss_finetuned=loss_tuned, val_loss_supervised=loss_gated,
+                               val_loss_finetuned=loss_tuned, ft_best_step=best["step"],
+                               val_loss_supervised=loss_gated,
                                chosen=choice, pairs_s=t_pairs, train_s=t_train,
                                finetune_s=t_ft),
                   evals=[])
