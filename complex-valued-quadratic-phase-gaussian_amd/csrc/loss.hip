@@ -374,6 +374,30 @@ __global__ __launch_bounds__(256) void k_target_stats(const float* __restrict__ 
   stats[((size_t)dc * P.Hv + q) * P.Wv + mc] = make_float2(mu, e);
 }
 
+// The target noise the streaming kernel draws in-kernel, as a tensor [D][C][H][W] of unit
+// normals: thread (x, row group g, plane-channel) makes the same Philox draw for pixels
+// (4 g .. 4 g + 3, x), so passing the tensor as `noise` gives the same loss and gradient
+// bit for bit.  grid (ceil(W / 256), ceil(H / 4), D C)
+__global__ __launch_bounds__(256) void k_loss_noise(unsigned long long seed,
+                                                    const int* __restrict__ step_counter,
+                                                    int C, int H, int W,
+                                                    float* __restrict__ noise) {
+  const int x = blockIdx.x * 256 + threadIdx.x, g = blockIdx.y, dc = blockIdx.z;
+  if (x >= W) return;
+  const int d = dc / C, c = dc % C;
+  const uint32_t step = step_counter ? (uint32_t)step_counter[0] : 0u;
+  const uint32_t hq = (uint32_t)((H + 3) / 4);
+  float z4[4];
+  normals4(seed, step, (uint32_t)d, ((uint32_t)c * hq + (uint32_t)g) * (uint32_t)W + (uint32_t)x,
+           z4);
+  float* np = noise + (size_t)dc * H * W;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int y = 4 * g + l;
+    if (y < H) np[(size_t)y * W + x] = z4[l];
+  }
+}
+
 // loss = sum_d [ mse_d / n + LAMBDA (1 - SSIM_d) ]; one workgroup, fixed summation order
 __global__ __launch_bounds__(1024) void k_loss_total(const float* __restrict__ ssim_part,
                                                      int n_ssim, const float* __restrict__ mse_part,
@@ -507,6 +531,20 @@ static LossParams loss_params(int channels, int planes, int height, int width,
   for (int d = 0; d < 16; ++d) P.noise_scale[d] = (noise_scale && d < planes) ? noise_scale[d] : 0.f;
   P.seed = seed;
   return P;
+}
+
+extern "C" int cvqpg_loss_noise(uint64_t seed, const int32_t* step_counter, int channels,
+                                int planes, int height, int width, float* noise, void* stream) {
+  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
+  CVQPG_CHECK_ARG(planes >= 1 && planes <= 16, "planes must be 1..16");
+  CVQPG_CHECK_ARG(height >= 1 && width >= 1, "invalid image size %dx%d", width, height);
+  CVQPG_CHECK_ARG(noise, "NULL argument pointer");
+  hipLaunchKernelGGL(k_loss_noise, dim3(ceil_div(width, 256), ceil_div(height, 4),
+                                        planes * channels),
+                     dim3(256), 0, (hipStream_t)stream, (unsigned long long)seed, step_counter,
+                     channels, height, width, noise);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
 }
 
 extern "C" size_t cvqpg_loss_target_stats_size(int channels, int planes, int height,

@@ -242,6 +242,15 @@ int cvqpg_loss_forward_backward(const float *intensity, const float *target, con
  * and gives the same gradient, bit for bit, as cvqpg_loss_forward_backward, and the same
  * loss up to the order in which its per-workgroup partial sums are added (target_stats
  * NULL: identical to it). */
+/* The unit normals the loss draws in-kernel when `noise` is NULL (Philox4x32-10 keyed by
+ * (seed, *step_counter, plane, pixel)), written as noise [D,C,H,W] float32.  Passed back
+ * as `noise` with the same noise_scale, they give the same loss and gradient bit for bit,
+ * so a caller can draw them ahead of the loss (or keep them for a replay).  No counterpart
+ * in the reference, whose multiplane_loss_odak draws torch.randn inside the loss
+ * (utils/propagator.py:417-449). */
+int cvqpg_loss_noise(uint64_t seed, const int32_t *step_counter, int channels, int planes,
+                     int height, int width, float *noise, void *stream);
+
 size_t cvqpg_loss_target_stats_size(int channels, int planes, int height, int width);
 int cvqpg_loss_target_stats(const float *target, int channels, int planes, int height,
                             int width, float *stats, size_t stats_bytes, void *stream);

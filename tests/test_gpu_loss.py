@@ -140,3 +140,29 @@ def test_target_statistics_path(pkg, C, D, H, W):
     err = max(float((st[..., 0] - mu).abs().max()), float((st[..., 1] - e2).abs().max()))
     print(f"case C={C} D={D} {H}x{W}: statistics max abs error {err:.1e}", flush=True)
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("C,D,H,W", [(3, 2, 42, 61), (1, 3, 1080, 1920)])
+def test_noise_tensor_matches_in_kernel_draw(pkg, C, D, H, W):
+    """cvqpg_loss_noise writes the normals the loss kernel draws itself: passed back as the
+    noise tensor they give the same loss and gradient bit for bit (rows not a multiple of 4
+    included)."""
+    from cvqpg_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(H + W)
+    inten = rng.uniform(-0.1, 1.2, (D, C, H, W)).astype(np.float32)
+    target = rng.uniform(0, 1, (D, C, H, W)).astype(np.float32)
+    mask = (rng.uniform(size=(D, C, H, W)) > 0.5).astype(np.float32)
+    scale = [1e-3 * float(target[d].max() - target[d].min()) for d in range(D)]
+    step = torch.full((1,), 7, dtype=torch.int32, device=DEV)
+    noise = torch.empty(D, C, H, W, device=DEV)
+    _lib.check(L.cvqpg_loss_noise(13, _lib.ptr(step), C, D, H, W, _lib.ptr(noise),
+                                  _lib.stream_handle()))
+    torch.cuda.synchronize()
+    z = noise.cpu().numpy()
+    l_in, g_in = run_gpu_loss(inten, target, mask, None, scale, C, D, H, W, seed=13, step=step)
+    l_t, g_t = run_gpu_loss(inten, target, mask, z, scale, C, D, H, W, seed=13, step=step)
+    print(f"case C={C} D={D} {H}x{W}: noise mean {z.mean():.4f} std {z.std():.4f}", flush=True)
+    assert torch.equal(g_in, g_t)
+    assert l_in == l_t
+    assert abs(z.mean()) < 0.05 and abs(z.std() - 1.0) < 0.05
