@@ -255,8 +255,8 @@ def bench_warmstart(args, dev, quick=False):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU runs "
-              "are not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the library "
+              "has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
     sys.path.insert(0, ROOT)

@@ -4,7 +4,7 @@ The reference starts every fit from random Gaussians (model_2d_gaussian_rgb.py:4
 and optimises them (train_2d_gaussian_rgb.py:187-211).  Here a small convolutional
 encoder-decoder predicts, from the target image and depth map, 8 Gaussians for every
 16x16 tile; those replace 8 x tiles of the random initial Gaussians, and the same fused
-fit then runs from there.  The plain initialisation stays the fallback and the judge:
+fit then runs from there.  The plain initialisation stays the fallback and the yardstick:
 ``time_to_psnr`` measures how long each start needs to reach the plain fit's final
 per-plane PSNR.
 
@@ -751,7 +751,7 @@ def experiment(device="cuda", train_size=(1920, 1080), n_train=16, fit_iters=600
     4. validation: the first-step loss on ``val_scenes`` further scenes picks the best
        fine-tuning checkpoint (every 250 steps) and then chooses between it (every slot)
        and the supervised network (present slots, gated phases); the plain start stays
-       the fallback and the judge;
+       the fallback and the yardstick;
     5. time-to-PSNR on held-out scenes of the chosen warm start against the plain start,
        and of the image-independent start from the mean slot encoding, which isolates
        what the network's conditioning on the image adds.
