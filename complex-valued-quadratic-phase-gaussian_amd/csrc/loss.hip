@@ -125,6 +125,8 @@ constexpr int RS_MIN = CVQPG_LOSS_RS_MIN;  // segments of at least ~RS_MIN rows:
 static_assert(RS % 4 == 0, "segments hold whole groups of four rows");
 constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 
+// waves per SIMD the register allocation must leave room for (1: no constraint; the
+// kernel reaches 4 with precomputed target statistics and 3 without, spill-free; 5 spills)
 #ifndef CVQPG_LOSS_WPE
 #define CVQPG_LOSS_WPE 1
 #endif
