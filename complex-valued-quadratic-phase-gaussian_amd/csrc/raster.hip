@@ -1103,27 +1103,48 @@ __global__ __launch_bounds__(256) void k_bwd_reduce_raw(
     Prologue pr, const int32_t* __restrict__ itr, int curv_warmup, int n,
     float* __restrict__ grads, float* __restrict__ sumsq) {
   __shared__ float red[256];
+  __shared__ float s_acc[256][REC + 1];
+  // 1. sums of the partial records, in key order: four lanes per Gaussian, lane q adding
+  //    floats 4q .. 4q + 3 of each record (a wave load covers 16 Gaussians' 64-byte
+  //    records), four records in flight per lane; 64 Gaussians per pass, four passes
+#pragma unroll 1
+  for (int pass = 0; pass < 4; ++pass) {
+    const int lg = pass * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+    const int gg = blockIdx.x * blockDim.x + lg;
+    float4 a4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (gg < n && !misc[1]) {
+      const int e0 = rect[gg].x;
+      const int e1 = e0 + (int)counts[gg];
+      for (int e = e0; e < e1; e += 4) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          v[u] = e + u < e1 ? partials[(size_t)(e + u) * (REC / 4) + q]
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (e + u < e1) {
+            a4.x += v[u].x;
+            a4.y += v[u].y;
+            a4.z += v[u].z;
+            a4.w += v[u].w;
+          }
+        }
+      }
+    }
+    s_acc[lg][4 * q] = a4.x;
+    s_acc[lg][4 * q + 1] = a4.y;
+    s_acc[lg][4 * q + 2] = a4.z;
+    s_acc[lg][4 * q + 3] = a4.w;
+  }
+  __syncthreads();
+  // 2. one thread per Gaussian from here on
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   float sq = 0.f;
   if (g < n) {
     float acc[REC];
 #pragma unroll
-    for (int i = 0; i < REC; ++i) acc[i] = 0.f;
-    if (!misc[1]) {
-      const int e0 = rect[g].x;
-      const int cnt = (int)counts[g];
-      for (int e = e0; e < e0 + cnt; ++e) {
-        const float4* src = partials + (size_t)e * (REC / 4);
-#pragma unroll
-        for (int i = 0; i < REC / 4; ++i) {
-          const float4 v = src[i];
-          acc[4 * i] += v.x;
-          acc[4 * i + 1] += v.y;
-          acc[4 * i + 2] += v.z;
-          acc[4 * i + 3] += v.w;
-        }
-      }
-    }
+    for (int i = 0; i < REC; ++i) acc[i] = s_acc[threadIdx.x][i];
     Act<C> a;
     activate<C>(R, g, pr, a);
     // gradients with respect to the rasterizer inputs
