@@ -197,6 +197,44 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restric
   if (i < n) out[i] += sums[i / SCAN_BLOCK];
 }
 
+// Second half of the exclusive scan when there are at most 1024 block sums: every
+// workgroup adds to its 256 elements the sum of the block sums before their SCAN_BLOCK
+// chunk, reducing them itself (so no separate pass scans the block sums); the first
+// workgroup, whose prefix is 0, reduces all of them for the total, the overflow flag, the
+// key count and the empty list's point_list[0] = 0.  grid ceil(n / 256)
+__global__ __launch_bounds__(256) void k_scan_finish(uint32_t* __restrict__ out,
+                                                     const uint32_t* __restrict__ sums, int nb,
+                                                     int64_t n, uint32_t* __restrict__ misc,
+                                                     int64_t cap, int32_t* __restrict__ num_pairs,
+                                                     int32_t* __restrict__ point_list0) {
+  static_assert(SCAN_BLOCK % 256 == 0, "a workgroup's elements lie in one chunk");
+  __shared__ uint32_t red[256];
+  const int t = threadIdx.x;
+  const int64_t first = (int64_t)blockIdx.x * 256;
+  const int chunk = (int)(first / SCAN_BLOCK);
+  const int lim = blockIdx.x == 0 ? nb : chunk;
+  uint32_t s = 0;
+  for (int k = t; k < lim; k += 256) s += sums[k];
+  red[t] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  const uint32_t pre = red[0];
+  const int64_t i = first + t;
+  if (blockIdx.x == 0) {
+    if (t == 0 && misc) {
+      misc[0] = pre;
+      misc[1] = (int64_t)pre > cap ? 1u : 0u;
+      if (num_pairs) *num_pairs = (int32_t)pre;
+      if (point_list0 && pre == 0) *point_list0 = 0;
+    }
+  } else if (i < n) {
+    out[i] += pre;
+  }
+}
+
 static int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t* sums, int64_t n,
                           uint32_t* misc, int64_t cap, int32_t* num_pairs, int32_t* pl0,
                           hipStream_t s) {
@@ -204,8 +242,15 @@ static int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t* sums, int
   CVQPG_CHECK_ARG(nb <= 1024 * 64, "scan of %lld elements exceeds the supported size",
                   (long long)n);
   hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, s, in, out, sums, n);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb, misc, cap, num_pairs, pl0);
-  hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, out, sums, n);
+  if (nb <= 1024) {
+    hipLaunchKernelGGL(k_scan_finish, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, out,
+                       sums, nb, n, misc, cap, num_pairs, pl0);
+  } else {
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb, misc, cap, num_pairs,
+                       pl0);
+    hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, out,
+                       sums, n);
+  }
   CVQPG_LAUNCH_CHECK();
   return 0;
 }
