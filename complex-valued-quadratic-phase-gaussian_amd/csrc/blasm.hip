@@ -500,16 +500,23 @@ __global__ __launch_bounds__(P::nt) void k_f1_rows_fx(const float2* __restrict__
       if (!G1::live(b)) continue;
       int j, col;
       G1::item(b, j, col);
+      // all loads of the row first (a load guarded together with its use would wait for
+      // each in turn); columns outside the padded field read the clamped edge
+      float2 hv[Fw::FIRST];
+#pragma unroll
+      for (int k = 0; k < Fw::FIRST; ++k) {
+        if (!in_mid_half<Fw>(k)) continue;
+        const int q = j + k * G1::CNT - o.zc0;
+        hv[k] = src[min(max(q - o.pad_w, 0), o.W - 1)];
+      }
 #pragma unroll
       for (int k = 0; k < Fw::FIRST; ++k) {
         float2 u = make_float2(0.f, 0.f);
         if (in_mid_half<Fw>(k)) {
           const int q = j + k * G1::CNT - o.zc0;
           if ((unsigned)q < (unsigned)o.pW) {
-            const int sc = min(max(q - o.pad_w, 0), o.W - 1);
             const float w = wy * taper(q, o.pW, o.pad_w, o.W);
-            const float2 h = src[sc];
-            u = glue(make_float2(h.x * w, h.y * w));
+            u = glue(make_float2(hv[k].x * w, hv[k].y * w));
           }
         }
         v[b][k] = u;
@@ -603,14 +610,26 @@ __global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ 
       if (!G1::live(b)) continue;
       int j, col;
       G1::item(b, j, col);
+      // every load of the row is issued before any is used (a load guarded together with
+      // its use would wait for each in turn); out-of-crop elements read element 0
+      float gi[Fw::FIRST];
+      float2 fo[Fw::FIRST];
+#pragma unroll
+      for (int k = 0; k < Fw::FIRST; ++k) {
+        if (!in_mid_half<Fw>(k)) continue;
+        const int x = j + k * G1::CNT - o.oc0;
+        const size_t e = ob + ((unsigned)x < (unsigned)o.ow ? x : 0);
+        gi[k] = grad_inten[e];
+        fo[k] = out[e];
+      }
 #pragma unroll
       for (int k = 0; k < Fw::FIRST; ++k) {
         float2 g = make_float2(0.f, 0.f);
         if (in_mid_half<Fw>(k)) {
           const int x = j + k * G1::CNT - o.oc0;
           if ((unsigned)x < (unsigned)o.ow) {
-            const float sc = 2.0f * grad_inten[ob + x] * o.norm;
-            const float2 f = fft::cmul(out[ob + x], crc);
+            const float sc = 2.0f * gi[k] * o.norm;
+            const float2 f = fft::cmul(fo[k], crc);
             g = make_float2(sc * f.x, sc * f.y);
           }
         }
@@ -671,10 +690,18 @@ __global__ __launch_bounds__(P::nt) void k_a1_rows_fx(const float2* __restrict__
     if (!GL::live(b)) continue;
     int j, col;
     GL::item(b, j, col);
+    // the field values of every output first (a load guarded together with its use would
+    // wait for each in turn); outputs outside the hologram read element 0
+    float2 fv[Iv::LAST];
 #pragma unroll
     for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {   // the hologram lies in [N/4, 3N/4)
       const int x = j + k * GL::CNT - c0;
-      if ((unsigned)x < (unsigned)o.W) grad_field[fb + x] = glue_adjoint(field[fb + x], v[b][k]);
+      fv[k] = field[fb + ((unsigned)x < (unsigned)o.W ? x : 0)];
+    }
+#pragma unroll
+    for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
+      const int x = j + k * GL::CNT - c0;
+      if ((unsigned)x < (unsigned)o.W) grad_field[fb + x] = glue_adjoint(fv[k], v[b][k]);
     }
   }
 }
