@@ -344,10 +344,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_hist(const uint32_t* __r
   __syncthreads();
   const int64_t P = misc[1] ? 0 : (int64_t)misc[0];
   const int64_t base = (int64_t)blockIdx.x * SORT_BLOCK;
-#pragma unroll 4
+  // all the block's keys are loaded before any is counted (a guarded load and its use
+  // would wait for each key in turn); keys past the end read element 0 and are not counted
+  uint32_t kv[SORT_ROUNDS];
+#pragma unroll
   for (int i = 0; i < SORT_ROUNDS; ++i) {
     const int64_t idx = base + i * SORT_THREADS + t;
-    if (idx < P) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+    kv[i] = keys[idx < P ? idx : 0];
+  }
+#pragma unroll
+  for (int i = 0; i < SORT_ROUNDS; ++i) {
+    const int64_t idx = base + i * SORT_THREADS + t;
+    if (idx < P) atomicAdd(&h[(kv[i] >> shift) & mask], 1u);
   }
   __syncthreads();
   hist[(int64_t)t * nb + blockIdx.x] = h[t];
@@ -371,12 +379,23 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
   for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
   const uint32_t mask = (1u << nbits) - 1u;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the block's keys and values are all loaded up front (one memory latency instead of
+  // one per round); entries past the end read element 0 and are not placed
+  uint32_t kk[SORT_ROUNDS], vv[SORT_ROUNDS];
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    const int64_t idx = blk + r * SORT_THREADS + t;
+    const int64_t ii = idx < P ? idx : 0;
+    kk[r] = keys_in[ii];
+    vv[r] = vals_in[ii];
+  }
   __syncthreads();
+#pragma unroll
   for (int r = 0; r < SORT_ROUNDS; ++r) {
     const int64_t idx = blk + r * SORT_THREADS + t;
     const bool valid = idx < P;
-    const uint32_t k = valid ? keys_in[idx] : 0u;
-    const uint32_t v = valid ? vals_in[idx] : 0u;
+    const uint32_t k = valid ? kk[r] : 0u;
+    const uint32_t v = valid ? vv[r] : 0u;
     const uint32_t d = (k >> shift) & mask;
     uint64_t m = __ballot(valid);
     for (int b = 0; b < nbits; ++b) {
