@@ -183,3 +183,45 @@ def render_backward(grad, means, inv_cov, colours, phase, opac, curv, area, k, d
     d_means[:, 0].index_add_(0, gi, dL_dm * 2.0 * (i00 * dx + i01 * dy))
     d_means[:, 1].index_add_(0, gi, dL_dm * 2.0 * (i11 * dy + i01 * dx))
     return d_means, d_inv, d_col, d_ph, d_op, d_cv, d_ar
+
+
+def render_binned_window(means, inv_cov, colours, phase, opac, curv, area, k, dxy2,
+                         width, height, bins, x0, y0, w, h, chunk_tiles=64):
+    """The binned forward (cuda/forward.cu:62-162) in FP64 on the window
+    [y0, y0 + h) x [x0, x0 + w) of a width x height frame, from the frame's key list
+    ``bins`` (the tiles overlapping the window are expanded, a few at a time, so a
+    1080p frame with 150k Gaussians fits in memory).  Returns complex128 [C, h, w]."""
+    means, inv_cov, colours, phase = _t(means), _t(inv_cov), _t(colours), _t(phase)
+    opac, curv, area, k = _t(opac), _t(curv), _t(area), _t(k)
+    C = colours.shape[1]
+    gx, gy = binning.tile_grid(width, height)
+    rng = bins["ranges"]
+    pl = bins["point_list"]
+    tiles = [ty * gx + tx for ty in range(y0 // 16, min(gy, (y0 + h + 15) // 16))
+             for tx in range(x0 // 16, min(gx, (x0 + w + 15) // 16))]
+    out = torch.zeros(C, h * w, dtype=torch.complex128)
+    ly, lx = torch.meshgrid(torch.arange(16), torch.arange(16), indexing="ij")
+    ly, lx = ly.reshape(1, -1), lx.reshape(1, -1)
+    for t0 in range(0, len(tiles), chunk_tiles):
+        ts = tiles[t0:t0 + chunk_tiles]
+        cnt = np.array([rng[t, 1] - rng[t, 0] for t in ts], dtype=np.int64)
+        if cnt.sum() == 0:
+            continue
+        ids = torch.as_tensor(np.concatenate([pl[rng[t, 0]:rng[t, 1]] for t in ts])
+                              .astype(np.int64))
+        tile_of = torch.as_tensor(np.repeat(np.array(ts, dtype=np.int64), cnt))
+        pxi = ((tile_of % gx)[:, None] * 16 + lx).reshape(-1)
+        pyi = ((tile_of // gx)[:, None] * 16 + ly).reshape(-1)
+        gi = ids[:, None].expand(-1, 256).reshape(-1)
+        keep = (pxi >= x0) & (pxi < min(x0 + w, width)) & (pyi >= y0) & (pyi < min(y0 + h, height))
+        pxi, pyi, gi = pxi[keep], pyi[keep], gi[keep]
+        dx = means[gi, 0] - pxi.to(F64)
+        dy = means[gi, 1] - pyi.to(F64)
+        _, _, _, alpha, base = pair_terms(dx, dy, inv_cov[gi, 0], inv_cov[gi, 1],
+                                          inv_cov[gi, 2], opac[gi], curv[gi], area[gi], dxy2)
+        pid = (pyi - y0) * w + (pxi - x0)
+        for c in range(C):
+            th = phase[gi, c] + k[c] * base
+            amp = colours[gi, c] * alpha
+            out[c].index_add_(0, pid, torch.complex(amp * torch.cos(th), amp * torch.sin(th)))
+    return out.view(C, h, w)

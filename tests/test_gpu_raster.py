@@ -40,7 +40,9 @@ CASES = [  # (n, W, H, C, seed)
     (500, 100, 37, 3, 1),      # ragged tiles on both axes
     (300, 17, 200, 1, 2),      # single tile column, grayscale
     (1000, 256, 256, 1, 3),    # 256 tiles -> one 8-bit radix pass
-    (3000, 300, 170, 3, 4),    # 19 x 11 tiles -> two radix passes
+    (3000, 300, 170, 3, 4),    # 19 x 11 = 209 tiles -> one 8-bit radix pass
+    (2500, 272, 256, 3, 6),    # 17 x 16 = 272 tiles -> 9 bits: passes of 8 and 1 bits
+    (3000, 320, 240, 1, 7),    # 20 x 15 = 300 tiles -> 9 bits
 ]
 
 
@@ -162,3 +164,81 @@ def test_rejects_bad_channels(pkg):
         qpf_render(g["means"], g["inv_cov"], g["colours"][:, :2].contiguous(),
                    g["phase"][:, :2].contiguous(), g["opac"], g["curv"], g["area"], g["radii"],
                    g["k"][:2], d["dxy2"], 32, 32)
+
+
+# ---------------------------------------------------------------------------------------
+# Full-size key lists: the radix sort runs ceil(log2 T / 8) passes, so these cases take
+# the multi-pass path that the small frames above never reach.
+# ---------------------------------------------------------------------------------------
+def workload_inputs(name, seed=0):
+    """Activated raster inputs of a named synthetic workload (model_2d_gaussian_rgb.py:83-153,
+    424-439, evaluated in FP64 by the oracle and rounded once to FP32)."""
+    from cvqpg_amd import synthetic
+    from oracle import model as omodel
+    spec = synthetic.WORKLOADS[name]
+    W, H, N = spec["W"], spec["H"], spec["N"]
+    raw = synthetic.make_scene(N, W, H, spec["wavelengths"], spec["pitch"], spec["kappa"], seed)
+    ri = omodel.raster_inputs({k: torch.as_tensor(v).double() for k, v in raw.items()}, (W, H),
+                              spec["wavelengths"], spec["pitch"])
+    f32 = lambda t: np.ascontiguousarray(t.detach().numpy(), dtype=np.float32)
+    d = dict(means=f32(ri["means"]), inv_cov=f32(ri["inv_cov"]), colours=f32(ri["colours"]),
+             phase=f32(ri["phase"]), opac=f32(ri["opac"]), curv=f32(ri["curv"]),
+             area=f32(ri["area"]), radii=ri["radii"].numpy().astype(np.int32),
+             k=f32(ri["k"]), dxy2=ri["dxy2"])
+    return d, W, H
+
+
+def check_bins(b, ref, label):
+    T = ref["ranges"].shape[0]
+    print(f"{label}: T={T} tiles ({max(1, (T - 1).bit_length())} key bits), P={ref['P']}",
+          flush=True)
+    assert b.num_pairs == ref["P"]
+    got_pl = b.point_list.cpu().numpy()
+    bad = np.flatnonzero(got_pl != ref["point_list"])
+    assert bad.size == 0, f"{bad.size} point_list entries differ, first at {bad[:5]}"
+    got_r = b.ranges.cpu().numpy()
+    bad = np.flatnonzero((got_r != ref["ranges"]).any(1))
+    assert bad.size == 0, f"{bad.size} tile ranges differ, first tiles {bad[:5]}"
+
+
+@pytest.mark.parametrize("name", ["rgb1080p_150k", "rgb4k_600k"])
+def test_binning_bit_exact_full_size(pkg, name):
+    """The 1920x1080 / 150k scene (120 x 68 = 8,160 tiles, 13 key bits, two radix passes) and
+    the 3840x2160 / 600k scene (240 x 135 = 32,400 tiles, 15 bits): point_list and ranges
+    equal the reference's 64-bit key sort (cuda/raster.cu:85-93) bit for bit."""
+    from cvqpg_amd import rasterizer
+    d, W, H = workload_inputs(name)
+    b = rasterizer.bin_gaussians(torch.as_tensor(d["means"]).to(DEV),
+                                 torch.as_tensor(d["radii"]).to(DEV), W, H, 3)
+    ref = binning.bin_gaussians(d["means"], d["radii"], W, H)
+    check_bins(b, ref, name)
+
+
+def test_binning_bit_exact_three_passes(pkg):
+    """256 x 257 = 65,792 tiles (17 key bits): radix passes of 8, 8 and 1 bits."""
+    from cvqpg_amd import rasterizer
+    n, W, H = 40_000, 4096, 4112
+    d = make_inputs(n, W, H, 1, 8)
+    b = rasterizer.bin_gaussians(torch.as_tensor(d["means"]).to(DEV),
+                                 torch.as_tensor(d["radii"]).to(DEV), W, H, 1)
+    ref = binning.bin_gaussians(d["means"], d["radii"], W, H)
+    check_bins(b, ref, "4096x4112")
+
+
+def test_field_full_size_window_matches_fp64(pkg):
+    """The 1920x1080 / 150k field, rendered whole on the GPU, on a 512 x 512 window against
+    the FP64 binned sum (cuda/forward.cu:62-162) of the same key list: <= 1e-4 relative L2."""
+    from cvqpg_amd import qpf_render
+    d, W, H = workload_inputs("rgb1080p_150k")
+    g = to_dev(d)
+    out = qpf_render(g["means"], g["inv_cov"], g["colours"], g["phase"], g["opac"], g["curv"],
+                     g["area"], g["radii"], g["k"], d["dxy2"], W, H)
+    torch.cuda.synchronize()
+    bins = binning.bin_gaussians(d["means"], d["radii"], W, H)
+    x0, y0, w, h = 704, 288, 512, 512
+    ref = raster.render_binned_window(*(d[k] for k in NAMES), d["k"], d["dxy2"], W, H, bins,
+                                      x0, y0, w, h)
+    e = rel(out[:, y0:y0 + h, x0:x0 + w].detach(), ref)
+    print(f"1080p/150k field, window {w}x{h} at ({x0},{y0}): rel L2 vs FP64 binned {e:.2e}",
+          flush=True)
+    assert e <= 1e-5

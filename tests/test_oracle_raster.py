@@ -130,3 +130,13 @@ def test_adjoint_dot_product():
     JTw = raster.render_backward(w, *_args(d), d["k"], d["dxy2"], W, H, bins)[2]
     rhs = float((JTw.numpy() * v).sum())
     assert abs(lhs - rhs) < 1e-10 * max(1, abs(lhs))
+
+
+def test_window_render_equals_binned_crop():
+    """render_binned_window on a window not aligned to tiles equals the crop of the whole
+    binned render (same pairs, FP64)."""
+    d = make_inputs(300, 80, 70, 3, 4)
+    full, bins = raster.render_binned(*_args(d), d["radii"], d["k"], d["dxy2"], 80, 70)
+    win = raster.render_binned_window(*_args(d), d["k"], d["dxy2"], 80, 70, bins, 21, 9, 50, 61,
+                                      chunk_tiles=3)
+    np.testing.assert_allclose(win.numpy(), full[:, 9:70, 21:71].numpy(), rtol=1e-12, atol=1e-14)
