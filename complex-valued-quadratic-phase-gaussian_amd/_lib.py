@@ -51,7 +51,10 @@ SIGNATURES = {
     "cvqpg_blasm_backward": (_c_int, [_p, _p, _p] + [_c_int] * 6 + [_p, _p, _c_f64, _c_f64, _c_int,
                                                                     _p, _c_size, _p, _p]),
     "cvqpg_blasm_reconstruct": (_c_int, [_p, _p] + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64, _c_int,
-                                                                   _p, _c_size, _p, _p]),
+                                                                   _p, _c_size, _p, _p, _p]),
+    "cvqpg_blasm_reconstruct_backward": (_c_int, [_p] * 4 + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64,
+                                                                            _c_int, _p, _c_size,
+                                                                            _p, _p, _p]),
     "cvqpg_loss_workspace_size": (_c_size, [_c_int] * 4),
     "cvqpg_loss_forward_backward": (_c_int, [_p, _p, _p, _p, _p, ctypes.c_uint64, _p, _c_int,
                                              _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),
@@ -61,13 +64,20 @@ SIGNATURES = {
     "cvqpg_loss_forward_backward_stats": (_c_int, [_p, _p, _p, _p, _p, _p, ctypes.c_uint64, _p,
                                                    _c_int, _c_int, _c_int, _c_int, _p, _c_size,
                                                    _p, _p, _p]),
+    "cvqpg_plane_l2_workspace_size": (_c_size, [_c_i64]),
+    "cvqpg_plane_l2": (_c_int, [_p, _p, _p, _p, _c_f32, _c_f32, _c_f32, _c_f32, _c_i64, _p, _c_size,
+                                _p, _p, _p]),
+    "cvqpg_ssim_term": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),
     "cvqpg_prologue_forward": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _c_f32, _c_f32,
                                         _c_int, _c_f32, _p, _c_size, _c_i64, _p, _p, _p]),
     "cvqpg_raster_backward_raw": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_f32, _c_f32,
                                            _c_int, _c_f32, _p, _c_int, _p, _c_size, _c_i64, _p,
                                            _p, _p]),
     "cvqpg_adan_step": (_c_int, [_p, _p, _p, _p, _p, _c_int, _p, _c_f32, _c_f32, _c_int, _c_f32,
-                                 _c_f32, _c_f32, _c_f32, _c_f32, _c_int, _c_int, _p, _p]),
+                                 _c_f32, _c_f32, _c_f32, _c_f32, _c_int, _c_int, _p, _p, _p]),
+    "cvqpg_raster_overflow_flags": (_c_int, [_p, _c_size, _c_int, _c_i64,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "cvqpg_raster_clear_overflow": (_c_int, [_p, _c_size, _c_int, _c_i64, _p]),
     "cvqpg_soft_window_pad": (_c_int, [_p] + [_c_int] * 5 + [_p, _p]),
     "cvqpg_dpac_encode": (_c_int, [_p] + [_c_int] * 3 + [_p, _p]),
     "cvqpg_metrics_workspace_size": (_c_size, [_c_int] * 4),

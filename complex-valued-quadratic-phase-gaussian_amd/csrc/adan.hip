@@ -14,6 +14,11 @@
 // replays correctly: the learning rate of the means group, the bias corrections and the
 // curvature warm-up are all derived from it inside the kernel, and k_advance bumps it
 // at the end of the step.
+//
+// A step whose keys overflowed the raster's capacity (the flag cvqpg_raster_overflow_flags
+// points at) rendered nothing: with that flag passed as skip_flag, the update and the
+// iteration advance are both skipped, so a replayed graph never applies an update computed
+// from an empty field.
 #include "common.h"
 #include "cvqpg.h"
 
@@ -35,9 +40,11 @@ struct AdanParams {
 __global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restrict__ grads,
                                               float* __restrict__ state,
                                               const float* __restrict__ sumsq, int n_partials,
-                                              const int32_t* __restrict__ iteration) {
+                                              const int32_t* __restrict__ iteration,
+                                              const uint32_t* __restrict__ skip) {
   __shared__ double red[256];
   __shared__ float s_coef;
+  if (skip && skip[0]) return;  // uniform over the grid
   const int t = threadIdx.x;
   double acc = 0.0;
   for (int i = t; i < n_partials; i += 256) acc += (double)sumsq[i];
@@ -99,7 +106,9 @@ __global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restr
   }
 }
 
-__global__ void k_advance(int32_t* iteration) { iteration[0] += 1; }
+__global__ void k_advance(int32_t* iteration, const uint32_t* skip) {
+  if (!(skip && skip[0])) iteration[0] += 1;
+}
 
 }  // namespace cvqpg
 
@@ -110,7 +119,7 @@ extern "C" int cvqpg_adan_step(float* const* params, const int64_t* sizes, const
                                const float* group_lr, float lr0, float eta_min, int num_itrs,
                                float beta1, float beta2, float beta3, float eps, float max_norm,
                                int curv_warmup, int curv_enabled, int32_t* iteration,
-                               void* stream) {
+                               const uint32_t* skip_flag, void* stream) {
   CVQPG_CHECK_ARG(params && sizes && grads && state && sumsq_partials && group_lr && iteration,
                   "NULL argument pointer");
   AdanParams A;
@@ -136,8 +145,8 @@ extern "C" int cvqpg_adan_step(float* const* params, const int64_t* sizes, const
   const int64_t total = A.off[7];
   if (total > 0)
     hipLaunchKernelGGL(k_adan, dim3(std::min<int64_t>(ceil_div(total, 256), 1024)), dim3(256), 0,
-                       s, A, grads, state, sumsq_partials, n_partials, iteration);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, iteration);
+                       s, A, grads, state, sumsq_partials, n_partials, iteration, skip_flag);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, iteration, skip_flag);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }

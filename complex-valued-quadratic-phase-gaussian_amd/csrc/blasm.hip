@@ -321,6 +321,35 @@ __global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G
     grad_field[fb + x] = glue_adjoint(field[fb + x], buf[fft::lidx(c0 + x)]);
 }
 
+// A1 for the propagator interface (adjoint of k_f1_rows_polar): unnormalised inverse FFT
+// along the row, then, with g = dL/dRe u + i dL/dIm u at u = a e^{i phi},
+//   dL/dphi = Re(conj(g) i u) = g.y u.x - g.x u.y,   dL/da = g.x cos phi + g.y sin phi.
+// grid (pH, C)
+template <class F>
+__global__ __launch_bounds__(ROW_NT) void k_a1_rows_polar(const float2* __restrict__ G,
+                                                          const float* __restrict__ phase,
+                                                          const float* __restrict__ amp,
+                                                          Optics o, fft::Plan plan,
+                                                          float* __restrict__ grad_phase,
+                                                          float* __restrict__ grad_amp) {
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const int y = blockIdx.x, c = blockIdx.y;
+  const float2* src = G + ((size_t)c * o.H + y) * o.nv;
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
+  __syncthreads();
+  F::template exec<-1, 1, ROW_NT>(buf, plan, stw);
+  const size_t base = ((size_t)c * o.pH + y) * o.pW;
+  for (int x = threadIdx.x; x < o.pW; x += ROW_NT) {
+    const float2 g = buf[fft::lidx(o.zc0 + x)];
+    float s, co;
+    sincosf(phase[base + x], &s, &co);
+    const float a = amp[base + x];
+    grad_phase[base + x] = a * (g.y * co - g.x * s);
+    grad_amp[base + x] = g.x * co + g.y * s;
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // column passes (CG adjacent columns per workgroup, XCD-aware block order)
 // ---------------------------------------------------------------------------------
@@ -1314,8 +1343,8 @@ extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitud
                                        int pad_height, int pad_width, int planes,
                                        const double* wavelengths, const double* distances,
                                        double pixel_pitch, double aperture_radius, int grid,
-                                       void* ws, size_t ws_bytes, float* intensity,
-                                       void* stream) {
+                                       void* ws, size_t ws_bytes, float* out_field,
+                                       float* intensity, void* stream) {
   Optics o;
   if (int rc = make_optics(o, channels, pad_height, pad_width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 1))
@@ -1329,5 +1358,43 @@ extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitud
   if (int rc = plans(o, w, pr, pc, s)) return rc;
   ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, phase, amplitude, o, pc, w.X);
   CVQPG_LAUNCH_CHECK();
-  return forward_cols_rows(w, o, pr, pc, nullptr, intensity, s);
+  return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
+}
+
+extern "C" int cvqpg_blasm_reconstruct_backward(
+    const float* phase, const float* amplitude, const float* grad_intensity,
+    const float* out_field, int channels, int pad_height, int pad_width, int planes,
+    const double* wavelengths, const double* distances, double pixel_pitch,
+    double aperture_radius, int grid, void* ws, size_t ws_bytes, float* grad_phase,
+    float* grad_amplitude, void* stream) {
+  Optics o;
+  if (int rc = make_optics(o, channels, pad_height, pad_width, pad_height, pad_width, planes,
+                           wavelengths, distances, pixel_pitch, aperture_radius, grid, 1))
+    return rc;
+  CVQPG_CHECK_ARG(phase && amplitude && grad_intensity && out_field && grad_phase &&
+                      grad_amplitude && ws,
+                  "NULL argument pointer");
+  const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
+  fft::Plan pr, pc;
+  if (int rc = plans(o, w, pr, pc, s)) return rc;
+  const float2* of = reinterpret_cast<const float2*>(out_field);
+  const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
+    using P = decltype(pl);
+    launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, grad_intensity,
+                 of, o, pc, w.Y);
+  });
+  if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
+  switch (o.cg) {
+    case 8: launch_a2<8>(w, o, pr, s); break;
+    case 4: launch_a2<4>(w, o, pr, s); break;
+    case 2: launch_a2<2>(w, o, pr, s); break;
+    default: launch_a2<1>(w, o, pr, s); break;
+  }
+  ROW_LAUNCH(k_a1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, (const float2*)w.X, phase,
+             amplitude, o, pc, grad_phase, grad_amplitude);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
 }

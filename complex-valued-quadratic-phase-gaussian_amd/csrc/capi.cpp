@@ -19,4 +19,4 @@ const char* get_error() { return g_err; }
 
 extern "C" const char* cvqpg_last_error(void) { return cvqpg::get_error(); }
 
-extern "C" int cvqpg_abi_version(void) { return 1; }
+extern "C" int cvqpg_abi_version(void) { return 2; }

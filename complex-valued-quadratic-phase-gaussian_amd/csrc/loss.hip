@@ -133,7 +133,9 @@ constexpr int NIN = NTH + WIN - 1;    // input columns staged per row
 // TS: the target's window statistics (mu_t, E[t^2]) of every map entry come precomputed
 // (k_target_stats, bit-identical to filtering them here); only the three moments of the
 // intensity (x, x^2, x t) are then filtered per step.
-template <bool TS>
+// L2 = false leaves the three L2 terms (and the mask and noise) out: the SSIM term alone,
+// GaussianLoss of utils/data_utils.py:22-41.
+template <bool TS, bool L2 = true>
 __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
     const float* __restrict__ inten, const float* __restrict__ target,
     const float* __restrict__ mask, const float* __restrict__ noise,
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       // 5. pixel row p = q: transposed vertical filter over map rows p - a (slots k - a)
       const int pr = q;
       if (pr < p0 || pr >= p_end) continue;
-      if (ns != 0.f && !noise && ((pr - p0) & 3) == 0 && pix_col)
+      if (L2 && ns != 0.f && !noise && ((pr - p0) & 3) == 0 && pix_col)
         normals4(P.seed, step, (uint32_t)d,
                  ((uint32_t)c * hq + (uint32_t)(pr >> 2)) * (uint32_t)Wd + (uint32_t)mc, z4);
       if (!pix_col) continue;
@@ -318,17 +320,20 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
         gC = fmaf(w, gring[sl][2], gC);
       }
       const float I = clamp01(cI);
-      float nz = 0.f;
-      if (ns != 0.f) {
-        const int l = (pr - p0) & 3;
-        nz = noise ? cN : (l == 0 ? z4[0] : l == 1 ? z4[1] : l == 2 ? z4[2] : z4[3]);
-        nz *= ns;
+      float gM = 0.f;
+      if constexpr (L2) {
+        float nz = 0.f;
+        if (ns != 0.f) {
+          const int l = (pr - p0) & 3;
+          nz = noise ? cN : (l == 0 ? z4[0] : l == 1 ? z4[1] : l == 2 ? z4[2] : z4[3]);
+          nz *= ns;
+        }
+        const float Tn = cT + nz;
+        const float e1 = I - cT, e2 = I - Tn;
+        const float wgt = cM * cM + Tn * Tn;
+        msum += e1 * e1 + wgt * e2 * e2;
+        gM = 2.f * P.inv_n * (e1 + wgt * e2);
       }
-      const float Tn = cT + nz;
-      const float e1 = I - cT, e2 = I - Tn;
-      const float wgt = cM * cM + Tn * Tn;
-      msum += e1 * e1 + wgt * e2 * e2;
-      const float gM = 2.f * P.inv_n * (e1 + wgt * e2);
       const float gS = gA + 2.f * I * gB + cT * gC;
       grp[pr * Wd + mc] = (cI >= 0.f && cI <= 1.f) ? gM + gS : 0.f;
     }
@@ -615,4 +620,27 @@ extern "C" int cvqpg_loss_forward_backward(const float* intensity, const float* 
                                            noise_scale, seed, step_counter, channels, planes,
                                            height, width, ws, ws_bytes, grad_intensity, loss,
                                            stream);
+}
+
+extern "C" int cvqpg_ssim_term(const float* image, const float* target, int channels,
+                               int height, int width, void* ws, size_t ws_bytes,
+                               float* grad_image, float* loss, void* stream) {
+  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
+  CVQPG_CHECK_ARG(height >= WIN && width >= WIN, "image smaller than the SSIM window");
+  CVQPG_CHECK_ARG(image && target && grad_image && loss && ws, "NULL argument pointer");
+  const LossWS need = carve_loss(nullptr, channels, 1, height, width);
+  CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
+  hipStream_t s = (hipStream_t)stream;
+  const LossWS w = carve_loss(ws, channels, 1, height, width);
+  LossParams P = loss_params(channels, 1, height, width, nullptr, 0);
+  loss_split(0, channels, height, width, P.own, P.rs);
+  const dim3 g(ceil_div(width, P.own), ceil_div(height, P.rs), channels);
+  const int nb = (int)(g.x * g.y * g.z);
+  // the mask operand is never read without the L2 terms; the target stands in for it
+  hipLaunchKernelGGL((k_loss_stream<false, false>), g, dim3(NTH), 0, s, image, target, target,
+                     nullptr, nullptr, nullptr, P, grad_image, w.ssim_part, w.mse_part);
+  hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(1024), 0, s, w.ssim_part, nb, w.mse_part, nb,
+                     P, loss);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
 }

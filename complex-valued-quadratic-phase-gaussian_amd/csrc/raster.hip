@@ -65,7 +65,8 @@ struct RasterWS {
   uint32_t* counts;     // [n]        tiles touched
   uint32_t* offs;       // [n]        exclusive prefix of counts
   uint32_t* sums_n;     // scan block sums of the count scan
-  uint32_t* misc;       // [0] total pairs, [1] overflow flag
+  uint32_t* misc;       // [0] total pairs, [1] overflow flag of the last count,
+                        // [2] overflowed counts since the last clear (sticky)
   // capacity-dependent regions
   uint32_t* keys_a;     // [cap]
   uint32_t* vals_a;     // [cap]
@@ -184,8 +185,10 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ sums,
       run += v;
     }
   if (t == 0 && misc) {
+    const uint32_t over = (int64_t)total > cap ? 1u : 0u;
     misc[0] = total;
-    misc[1] = (int64_t)total > cap ? 1u : 0u;
+    misc[1] = over;
+    misc[2] += over;
     if (num_pairs) *num_pairs = (int32_t)total;
     if (point_list0 && total == 0) *point_list0 = 0;
   }
@@ -225,8 +228,10 @@ __global__ __launch_bounds__(256) void k_scan_finish(uint32_t* __restrict__ out,
   const int64_t i = first + t;
   if (blockIdx.x == 0) {
     if (t == 0 && misc) {
+      const uint32_t over = (int64_t)pre > cap ? 1u : 0u;
       misc[0] = pre;
-      misc[1] = (int64_t)pre > cap ? 1u : 0u;
+      misc[1] = over;
+      misc[2] += over;
       if (num_pairs) *num_pairs = (int32_t)pre;
       if (point_list0 && pre == 0) *point_list0 = 0;
     }
@@ -418,6 +423,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(
     for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
     __syncthreads();
   }
+}
+
+__global__ void k_clear_overflow(uint32_t* __restrict__ misc) {
+  if (threadIdx.x < 3) misc[threadIdx.x] = 0u;
 }
 
 __global__ void k_clear_ranges(int32_t* __restrict__ ranges, int64_t n,
@@ -1319,6 +1328,23 @@ extern "C" size_t cvqpg_raster_workspace_size(int n, int channels, int width, in
   (void)width;
   (void)height;
   return carve(nullptr, n, capacity).bytes;
+}
+
+extern "C" int cvqpg_raster_overflow_flags(void* ws, size_t ws_bytes, int n, int64_t capacity,
+                                           uint32_t** flags) {
+  if (int rc = check_common(n, 1, 1, 1, ws, ws_bytes, capacity)) return rc;
+  CVQPG_CHECK_ARG(flags != nullptr, "flags is NULL");
+  *flags = carve(ws, n, capacity).misc + 1;  // host arithmetic only: no device access
+  return 0;
+}
+
+extern "C" int cvqpg_raster_clear_overflow(void* ws, size_t ws_bytes, int n, int64_t capacity,
+                                           void* stream) {
+  if (int rc = check_common(n, 1, 1, 1, ws, ws_bytes, capacity)) return rc;
+  hipLaunchKernelGGL(k_clear_overflow, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     carve(ws, n, capacity).misc);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
 }
 
 extern "C" int cvqpg_bin_count(const float* means, const int32_t* radii, int n, int width,

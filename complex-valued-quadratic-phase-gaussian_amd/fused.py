@@ -15,9 +15,14 @@ on the current stream:
 
 Nothing in ``step`` allocates or synchronises, so ``capture`` can record it into a HIP
 graph (torch.cuda.CUDAGraph drives hipStreamBeginCapture/EndCapture on ROCm).  The key
-buffers are sized once from an eager count with headroom; ``check_capacity`` reads the
-device key count back (one host read) so a caller can re-size if the scene grows.
+buffers are sized once from an eager count with headroom.  A step whose keys do not fit
+renders nothing, and its Adan update and iteration advance are skipped on the device
+(the raster workspace's overflow flag is the optimiser's skip flag), so a replayed graph
+never applies an update computed from an empty field; the workspace also counts such
+steps.  ``check_capacity`` reads the key count and that count back (one host read) so a
+caller can re-size with ``grow``.
 """
+import ctypes
 import math
 
 import torch
@@ -103,10 +108,21 @@ class FusedTrainer:
 
     # ------------------------------------------------------------------ sizing
     def _alloc_raster(self, capacity):
+        L = _lib.lib()
         self.capacity = int(capacity)
         nb = workspace_bytes(self.N, self.C, self.W, self.H, self.capacity)
         self.raster_ws = torch.empty(nb, dtype=torch.uint8, device=self.dev)
         self.point_list = torch.empty(max(self.capacity, 1), dtype=torch.int32, device=self.dev)
+        flags = ctypes.c_void_p()
+        _lib.check(L.cvqpg_raster_overflow_flags(_lib.ptr(self.raster_ws), nb, self.N,
+                                                 self.capacity, ctypes.byref(flags)))
+        self.overflow_flags = flags
+        off = flags.value - self.raster_ws.data_ptr()
+        # [key count, overflow of the last count, overflowed counts since the clear]
+        self._misc = self.raster_ws[off - 4:off + 8].view(torch.int32)
+        _lib.check(L.cvqpg_raster_clear_overflow(_lib.ptr(self.raster_ws), nb, self.N,
+                                                 self.capacity, self._s()))
+        self.overflowed_steps = 0
 
     def _size_keys(self, headroom):
         """Eager prologue + count with a provisional capacity, then fix the capacity."""
@@ -118,10 +134,13 @@ class FusedTrainer:
         self._alloc_raster(max(1024, int(P * headroom)))
 
     def check_capacity(self):
-        """Host read of the last key count; True if it still fits the buffers."""
-        P = int(self.num_pairs.item())
+        """One host read of the last key count and of the number of overflowed steps since
+        the buffers were sized; True if every step fitted (``overflowed_steps`` holds the
+        count: those steps changed neither the parameters nor the iteration)."""
+        P, _, over = (int(v) for v in self._misc.cpu())
         self.last_pairs = P
-        return P <= self.capacity
+        self.overflowed_steps = over
+        return P <= self.capacity and over == 0
 
     def grow(self, headroom=1.25):
         """Re-size the key buffers for the current parameters (optimiser state and the
@@ -195,7 +214,7 @@ class FusedTrainer:
                 _lib.ptr(self.state), _lib.ptr(self.sumsq), self.n_partials,
                 _lib.floats(self.group_lr), self.lr0, 1e-3, self.num_itrs, b1, b2, b3, 1e-8,
                 1.0, self.warmup, 1 if self.curv_mode != 0 else 0, _lib.ptr(self.iteration),
-                S()))),
+                self.overflow_flags, S()))),
         ]
 
     STAGES = ("prologue", "bin_count", "bin_sort", "render_fwd", "blasm_fwd", "loss",

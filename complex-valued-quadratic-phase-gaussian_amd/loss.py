@@ -10,13 +10,96 @@
   algorithm of pytorch-msssim 1.0.0, which is not installed here).
 * ``multiplane_loss`` -- factory of utils/data_utils.py:66-82.
 
-These are PyTorch implementations (setup work and the PyTorch baseline step); the
-fused training step evaluates the same loss in the gfx950 loss kernels.
+``multiplane_loss_odak.__call__`` and ``GaussianLoss`` run on the gfx950 kernels
+(``cvqpg_plane_l2``, ``cvqpg_ssim_term``, differentiable through torch.autograd.Function)
+when ``backend='hip'`` (the default) and in PyTorch when ``backend='torch'`` (the
+reference-structured baseline step).  There is no fallback between them: the HIP backend
+raises if the library is missing or the tensors are not on the GPU.  Target synthesis
+(``set_targets`` / ``add_defocus_blur``, once per fit) stays in PyTorch.  The fused
+training step evaluates the whole multi-plane loss in one pass of the loss kernel.
 """
 import math
 
 import torch
 import torch.nn.functional as F
+
+from . import _lib
+
+BACKEND = "hip"
+
+
+def _need_gpu(*ts):
+    for t in ts:
+        if not t.is_cuda:
+            raise RuntimeError("the HIP loss kernels need GPU tensors (use backend='torch' on "
+                               "the CPU)")
+
+
+class _PlaneL2(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, image, target, mask, noise, noise_ratio, weights):
+        _need_gpu(image, target, mask)
+        im = image.detach().contiguous().float()
+        tg = target.detach().contiguous().float()
+        mk = mask.detach().contiguous().float()
+        if not (im.numel() == tg.numel() == mk.numel()):
+            raise ValueError(f"image, target and mask must have the same number of elements "
+                             f"({im.numel()}, {tg.numel()}, {mk.numel()})")
+        nz = None if noise is None else noise.detach().contiguous().float()
+        L = _lib.lib()
+        n = im.numel()
+        ws = torch.empty(int(L.cvqpg_plane_l2_workspace_size(n)), dtype=torch.uint8,
+                         device=im.device)
+        grad = torch.empty_like(im) if image.requires_grad else None
+        out = torch.empty(1, device=im.device)
+        _lib.check(L.cvqpg_plane_l2(_lib.ptr(im), _lib.ptr(tg), _lib.ptr(mk), _lib.ptr(nz),
+                                    float(noise_ratio), float(weights[0]), float(weights[1]),
+                                    float(weights[2]), n, _lib.ptr(ws), ws.numel(),
+                                    _lib.ptr(grad), _lib.ptr(out),
+                                    _lib.stream_handle(im.device)))
+        ctx.save_for_backward(grad if grad is not None else out)
+        ctx.has_grad = grad is not None
+        ctx.shape = image.shape
+        return out[0]
+
+    @staticmethod
+    def backward(ctx, g):
+        if not ctx.has_grad:
+            return None, None, None, None, None, None
+        (grad,) = ctx.saved_tensors
+        return (grad * g).view(ctx.shape), None, None, None, None, None
+
+
+class _SSIMTerm(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pred, target):
+        _need_gpu(pred, target)
+        shape = pred.shape
+        if pred.dim() == 4:
+            if pred.shape[0] != 1:
+                raise ValueError("the HIP SSIM term takes one image [C, H, W] or [1, C, H, W]")
+            pred, target = pred[0], target[0]
+        im = pred.detach().contiguous().float()
+        tg = target.detach().contiguous().float()
+        C, H, W = im.shape
+        L = _lib.lib()
+        nb = int(L.cvqpg_loss_workspace_size(C, 1, H, W))
+        if nb == 0:
+            raise ValueError(f"image {H}x{W} is smaller than the 11x11 SSIM window")
+        ws = torch.empty(nb, dtype=torch.uint8, device=im.device)
+        grad = torch.empty_like(im)
+        out = torch.empty(1, device=im.device)
+        _lib.check(L.cvqpg_ssim_term(_lib.ptr(im), _lib.ptr(tg), C, H, W, _lib.ptr(ws),
+                                     ws.numel(), _lib.ptr(grad), _lib.ptr(out),
+                                     _lib.stream_handle(im.device)))
+        ctx.save_for_backward(grad)
+        ctx.shape = shape
+        return out[0]
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return (grad * g).view(ctx.shape), None
 
 SSIM_WIN, SSIM_SIGMA, SSIM_K = 11, 1.5, (0.01, 0.03)
 
@@ -54,7 +137,12 @@ def ssim(X, Y, data_range=1.0):
     return smap.flatten(2).mean(-1).mean()
 
 
-def GaussianLoss(pred, target, lambda_ssim=0.025):  # noqa: N802 (reference name)
+def GaussianLoss(pred, target, lambda_ssim=0.025, backend=None):  # noqa: N802 (reference name)
+    """utils/data_utils.py:22-41: lambda_ssim * (1 - SSIM(pred, target)) * 0.2."""
+    if (backend or BACKEND) == "hip":
+        if lambda_ssim != 0.025:
+            raise ValueError("the HIP SSIM term uses the reference's lambda_ssim = 0.025")
+        return _SSIMTerm.apply(pred, target)
     return lambda_ssim * (1 - ssim(pred, target)) * 0.2
 
 
@@ -64,8 +152,10 @@ GaussianLossGrayscale = GaussianLoss
 class multiplane_loss_odak:  # noqa: N801 (reference class name)
     def __init__(self, target_image, target_depth, blur_ratio=0.25, target_blur_size=10,
                  number_of_planes=4, weights=(1.0, 2.1, 0.6), multiplier=1.0, scheme="defocus",
-                 reduction="mean", split_ratio=1.0, device=torch.device("cpu")):
+                 reduction="mean", split_ratio=1.0, device=torch.device("cpu"), backend=None):
         self.device = torch.device(device)
+        self.backend = backend or BACKEND
+        self._draws = 0
         self.target_image = target_image.float().to(self.device)
         self.target_depth = target_depth.float().to(self.device)
         self.target_blur_size = target_blur_size + (1 if target_blur_size % 2 == 0 else 0)
@@ -118,9 +208,16 @@ class multiplane_loss_odak:  # noqa: N801 (reference class name)
 
     def __call__(self, image, target, plane_id=None, inject_noise=False, noise_ratio=1e-3, noise=None):
         """utils/propagator.py:417-449; ``noise`` (optional) replaces the internal
-        randn draw so a caller can pin the perturbation."""
-        l2 = self.weights[0] * F.mse_loss(image, target)
+        randn draw so a caller can pin the perturbation.  With the HIP backend the draw
+        is the Philox4x32-10 normal of the loss kernels (cvqpg_loss_noise), keyed by a
+        per-object call counter."""
         mask = self.masks if plane_id is None else self.masks[plane_id]
+        if self.backend == "hip":
+            if inject_noise and noise is None:
+                noise = self._draw(target)
+            return _PlaneL2.apply(image, target, mask, noise if inject_noise else None,
+                                  noise_ratio, self.weights[:3])
+        l2 = self.weights[0] * F.mse_loss(image, target)
         if inject_noise:
             if noise is None:
                 noise = torch.randn_like(target)
@@ -128,6 +225,21 @@ class multiplane_loss_odak:  # noqa: N801 (reference class name)
         l2_mask = self.weights[1] * F.mse_loss(image * mask, target * mask)
         l2_cor = self.weights[2] * F.mse_loss(image * target, target * target)
         return l2 + l2_mask + l2_cor
+
+    def _draw(self, target):
+        """Standard normals shaped like ``target`` from the loss kernels' generator."""
+        _need_gpu(target)
+        t = target.reshape(-1, target.shape[-2], target.shape[-1])
+        C = 3 if t.shape[0] % 3 == 0 else 1
+        planes = t.shape[0] // C
+        if planes > 16:
+            raise ValueError(f"cannot draw noise shaped {tuple(target.shape)}")
+        out = torch.empty(t.shape, device=target.device)
+        self._draws += 1
+        _lib.check(_lib.lib().cvqpg_loss_noise(0x5EED + self._draws, None, C, planes,
+                                               t.shape[1], t.shape[2], _lib.ptr(out),
+                                               _lib.stream_handle(target.device)))
+        return out.view_as(target)
 
 
 def multiplane_loss(target_image, target_depth, args_prop):

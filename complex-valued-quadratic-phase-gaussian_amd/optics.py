@@ -66,18 +66,61 @@ class HologramOptics:
             _lib.stream_handle(dev)))
         return grad
 
-    def reconstruct_planes(self, phase, amplitude):
+    def _plane_common(self):
+        return (self.C, self.pH, self.pW, self.D, _lib.doubles(self.wavelengths),
+                _lib.doubles(self.distances), self.pitch, self.aperture, self.grid)
+
+    def reconstruct_planes(self, phase, amplitude, keep_field=False):
         """propagator.reconstruct for one frame per channel over the full padded plane:
-        phase, amplitude [C, pH, pW] -> intensity [D, C, pH, pW] (no gradient)."""
+        phase, amplitude [C, pH, pW] -> intensity [D, C, pH, pW], and with ``keep_field``
+        also the complex plane fields [D, C, pH, pW] the backward needs (else None)."""
         dev = phase.device
         ws = self._workspace(dev, 1)
         inten = torch.empty(self.D, self.C, self.pH, self.pW, device=dev)
+        out = torch.empty(self.D, self.C, self.pH, self.pW, dtype=torch.complex64, device=dev) \
+            if keep_field else None
         _lib.check(_lib.lib().cvqpg_blasm_reconstruct(
             _lib.ptr(phase.contiguous().float()), _lib.ptr(amplitude.contiguous().float()),
-            self.C, self.pH, self.pW, self.D, _lib.doubles(self.wavelengths),
-            _lib.doubles(self.distances), self.pitch, self.aperture, self.grid, _lib.ptr(ws),
-            ws.numel(), _lib.ptr(inten), _lib.stream_handle(dev)))
+            *self._plane_common(), _lib.ptr(ws), ws.numel(),
+            None if out is None else _lib.ptr(torch.view_as_real(out)), _lib.ptr(inten),
+            _lib.stream_handle(dev)))
+        return (inten, out) if keep_field else inten
+
+    def reconstruct_planes_backward(self, phase, amplitude, grad_intensity, out_field):
+        """Adjoint of ``reconstruct_planes``: (dL/dphase, dL/damplitude), each [C, pH, pW]."""
+        dev = phase.device
+        ws = self._workspace(dev, 1)
+        gp = torch.empty(self.C, self.pH, self.pW, device=dev)
+        ga = torch.empty_like(gp)
+        _lib.check(_lib.lib().cvqpg_blasm_reconstruct_backward(
+            _lib.ptr(phase.contiguous().float()), _lib.ptr(amplitude.contiguous().float()),
+            _lib.ptr(grad_intensity.contiguous().float()),
+            _lib.ptr(torch.view_as_real(out_field)), *self._plane_common(), _lib.ptr(ws),
+            ws.numel(), _lib.ptr(gp), _lib.ptr(ga), _lib.stream_handle(dev)))
+        return gp, ga
+
+
+class _ReconstructPlanes(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phase, amplitude, optics):
+        phase = phase.contiguous().float()
+        amplitude = amplitude.contiguous().float()
+        inten, out = optics.reconstruct_planes(phase, amplitude, keep_field=True)
+        ctx.save_for_backward(phase, amplitude, out)
+        ctx.optics = optics
         return inten
+
+    @staticmethod
+    def backward(ctx, grad_inten):
+        phase, amplitude, out = ctx.saved_tensors
+        gp, ga = ctx.optics.reconstruct_planes_backward(phase, amplitude, grad_inten, out)
+        return gp, ga, None
+
+
+def reconstruct_planes(phase, amplitude, optics):
+    """Differentiable phase, amplitude [C, pH, pW] -> intensities [D, C, pH, pW] over the
+    whole padded plane (one frame per channel)."""
+    return _ReconstructPlanes.apply(phase, amplitude, optics)
 
 
 class _Propagate(torch.autograd.Function):

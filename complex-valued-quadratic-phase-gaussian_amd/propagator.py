@@ -10,14 +10,20 @@ which is off by ~2e-3 rad rms at these sizes).  The frequency grid is odak's
 half-sample ``linspace(-1/(2dx) + 1/(4X), 1/(2dx) - 1/(4X), n)`` by default
 (``grid='odak'``) or the exact FFT bins (``grid='fft'``).
 
-``reconstruct`` runs on the gfx950 BLASM kernels of libcvqpg when
-``backend='hip'`` (the default once available) and on torch.fft with a cached
-transfer function when ``backend='torch'`` (the reference's own structure, kept as
-the PyTorch baseline).
+``reconstruct`` runs on the gfx950 BLASM kernels of libcvqpg when ``backend='hip'``
+(the default; differentiable, with the adjoint kernels behind a torch.autograd.Function,
+so ``no_grad=False`` trains through it as train_2d_gaussian_rgb.py:94 does) and on
+torch.fft with a cached transfer function when ``backend='torch'`` (the reference's own
+structure, kept as the PyTorch baseline).  There is no fallback between the two: the
+HIP backend raises if the library is missing.  A custom aperture tensor
+(utils/propagator.py:195-200, zero-padded to the 2x grid) is supported by the torch
+backend; the HIP kernels take circular apertures only and reject it.
 """
 import math
 
 import torch
+
+from . import optics as optics_mod
 
 
 def freq_grid(n, dx, grid="odak", device="cpu"):
@@ -82,14 +88,31 @@ class propagator:  # noqa: N801  (reference class name)
                               if laser_channel_power is None else torch.as_tensor(laser_channel_power))
         self.channel_power = self.channel_power.float().cpu()
         nu, nv = 2 * self.resolution[0], 2 * self.resolution[1]
+        self.custom_aperture = aperture is not None
         if aperture is None:
             if aperture_size is None:
                 aperture_size = max(self.resolution)
+            self.aperture_size = float(aperture_size)
             self.aperture = circular_binary_mask(nu, nv, aperture_size, device=self.device)
         else:
-            raise NotImplementedError("custom aperture tensors are not supported")
+            # utils/propagator.py:199-200: the aperture at the hologram's resolution,
+            # zero-padded (odak zero_pad, centred) to the 2x grid
+            a = torch.as_tensor(aperture, dtype=torch.float32, device=self.device)
+            pH, pW = self.resolution
+            if a.shape[-2:] != (pH, pW):
+                raise ValueError(f"aperture must be {pH}x{pW}, got {tuple(a.shape[-2:])}")
+            self.aperture = torch.zeros(nu, nv, device=self.device)
+            self.aperture[pH - pH // 2:pH - pH // 2 + pH, pW - pW // 2:pW - pW // 2 + pW] = \
+                a.reshape(pH, pW)
+            self.aperture_size = None
         self.aperture_is_open = bool(self.aperture.min() >= 1.0)
+        if backend not in ("hip", "torch"):
+            raise ValueError(f"backend must be 'hip' or 'torch', got {backend!r}")
+        if backend == "hip" and self.custom_aperture:
+            raise NotImplementedError("the HIP BLASM kernels take circular apertures only; "
+                                      "use backend='torch' for a custom aperture tensor")
         self._kernels = None
+        self._optics = None
 
     def distance_list(self):
         return [float(z) for z in self.distances.double()]
@@ -104,7 +127,9 @@ class propagator:  # noqa: N801  (reference class name)
                 for wl in self.wavelengths:
                     H = transfer_function(nu, nv, self.pixel_pitch, wl, z, self.grid, self.device)
                     if not self.aperture_is_open:
-                        H = H * self.aperture
+                        # custom() applies the aperture to the spectrum and to H
+                        # (utils/propagator.py:59-61): a factor aperture^2
+                        H = H * (self.aperture * self.aperture)
                     row.append(torch.fft.ifftshift(H))
                 ks.append(torch.stack(row))
             self._kernels = torch.stack(ks)
@@ -116,14 +141,53 @@ class propagator:  # noqa: N801  (reference class name)
                 for c in range(self.number_of_channels) if float(self.channel_power[f, c]) != 0]
 
     def reconstruct(self, hologram_phases, amplitude=None, no_grad=True):
+        """utils/propagator.py:224-246: hologram phases [F, pH, pW] (a leading batch dim
+        of 1 is squeezed) and amplitudes [C, pH, pW] -> intensities [F, D, C, pH, pW]."""
+        impl = self._reconstruct_hip if self.backend == "hip" else self._reconstruct_torch
         if no_grad:
             with torch.no_grad():
-                return self._reconstruct_torch(hologram_phases, amplitude)
-        return self._reconstruct_torch(hologram_phases, amplitude)
+                return impl(hologram_phases, amplitude)
+        return impl(hologram_phases, amplitude)
+
+    def optics(self):
+        """The HIP propagation plan of this propagator (full padded planes)."""
+        if self._optics is None:
+            pH, pW = self.resolution
+            self._optics = optics_mod.HologramOptics(
+                (pW, pH), (pH, pW), self.wavelengths, self.pixel_pitch, self.distance_list(),
+                aperture_size=self.aperture_size, grid=self.grid)
+        return self._optics
+
+    def _reconstruct_hip(self, phases, amplitude):
+        """utils/propagator.py:248-290 on the gfx950 kernels: one differentiable
+        cvqpg_blasm_reconstruct per frame, channel c lit with power[f, c] (unlit channels
+        contribute zero, as the reference's skip leaves them)."""
+        if phases.dim() > 3:
+            phases = phases.squeeze(0)
+        pH, pW = self.resolution
+        D, C, F = self.number_of_depth_layers, self.number_of_channels, self.number_of_frames
+        if amplitude is None:
+            amplitude = torch.ones(C, pH, pW, device=phases.device)
+        opt = self.optics()
+        power = self.channel_power
+        eye = power.shape == (C, C) and bool(torch.equal(power, torch.eye(C)))
+        if eye:      # frame c lights channel c only: one call for every frame
+            inten = optics_mod.reconstruct_planes(phases[:C], amplitude, opt)   # [D, C, pH, pW]
+            zero = torch.zeros_like(inten[:, 0])
+            return torch.stack([torch.stack([inten[:, c] if c == f else zero for c in range(C)],
+                                            dim=1) for f in range(F)])
+        frames = []
+        for f in range(F):
+            pw = power[f].to(phases.device, amplitude.dtype)
+            if bool((pw == 0).all()):
+                frames.append(torch.zeros(D, C, pH, pW, device=phases.device))
+                continue
+            ph = phases[f].unsqueeze(0).expand(C, pH, pW)
+            frames.append(optics_mod.reconstruct_planes(ph, amplitude * pw.view(C, 1, 1), opt))
+        return torch.stack(frames)
 
     def _reconstruct_torch(self, phases, amplitude):
-        """utils/propagator.py:248-290 on torch.fft; the aperture is folded into H
-        (a binary aperture applied twice equals applying it once)."""
+        """utils/propagator.py:248-290 on torch.fft; the aperture is folded into H."""
         if phases.dim() > 3:
             phases = phases.squeeze(0)
         pH, pW = self.resolution

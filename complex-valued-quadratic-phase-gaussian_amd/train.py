@@ -9,6 +9,7 @@
   Logging reads (``loss.item()``, the per-step PSNR) are left out of the step so it
   runs without host synchronisation; ``psnr`` computes it on demand.
 """
+import copy
 import math
 
 import torch
@@ -104,7 +105,13 @@ class AutogradTrainer:
     """One training iteration through autograd (train_2d_gaussian_rgb.py:187-211)."""
 
     def __init__(self, wl, num_itrs=2001, backend="torch", curvature_frozen=False):
+        """``backend`` selects the propagator and loss implementation: "hip" (the gfx950
+        kernels behind the reference's objects) or "torch" (the reference's PyTorch
+        structure, the baseline).  The rasterizer follows ``args_prop.renderer``."""
         self.wl = wl
+        self.backend = backend
+        self.loss_fn = copy.copy(wl.loss_fn)
+        self.loss_fn.backend = backend
         self.scene = Scene2D(wl.gaussians, wl.args_prop)
         make_trainable_2d(wl.gaussians)
         if curvature_frozen:
@@ -122,8 +129,8 @@ class AutogradTrainer:
             pred = intens[d][:, pH // 2 - H // 2:pH // 2 + H // 2, pW // 2 - W // 2:pW // 2 + W // 2]
             pred = torch.clamp(pred, 0.0, 1.0)
             nz = None if noises is None else noises[d]
-            total = total + self.wl.loss_fn(pred, self.wl.targets[d], d, inject_noise=True, noise=nz)
-            total = total + loss_mod.GaussianLoss(pred, self.wl.targets[d])
+            total = total + self.loss_fn(pred, self.wl.targets[d], d, inject_noise=True, noise=nz)
+            total = total + loss_mod.GaussianLoss(pred, self.wl.targets[d], backend=self.backend)
         total.backward()
         for p in self.params:
             if p.grad is not None:

@@ -53,6 +53,20 @@ int cvqpg_bin_count(const float *means, const int32_t *radii, int n, int width, 
                     void *ws, size_t ws_bytes, int64_t capacity, int32_t *num_pairs,
                     void *stream);
 
+/* Key-overflow flags of a raster workspace, for callers that size the key buffers once and
+ * replay a captured step (the reference reads P back on every call and allocates to fit,
+ * cuda/raster.cu:71-80, so it has no counterpart).  Host-only: *flags receives the DEVICE
+ * address of two uint32 inside ws: flags[0] = 1 if the last cvqpg_bin_count /
+ * cvqpg_qpf_render_forward found more keys than `capacity` (that call then renders nothing
+ * and its adjoint is zero), flags[1] = the number of such counts since the last
+ * cvqpg_raster_clear_overflow (sticky; no step clears it).  Pass flags as the skip_flag of
+ * cvqpg_adan_step so an overflowed step leaves parameters, state and iteration unchanged. */
+int cvqpg_raster_overflow_flags(void *ws, size_t ws_bytes, int n, int64_t capacity,
+                                uint32_t **flags);
+/* Zeroes both flags (and the stored key count) with a kernel on `stream`. */
+int cvqpg_raster_clear_overflow(void *ws, size_t ws_bytes, int n, int64_t capacity,
+                                void *stream);
+
 /* Emits the (tile, Gaussian) keys, sorts them by tile with a stable LDS radix
  * sort (ids ascend within a tile, so the result equals the reference's 64-bit
  * key sort bit for bit) and writes point_list [max(P,1)] (sorted Gaussian ids;
@@ -207,11 +221,25 @@ int cvqpg_blasm_backward(const float *field, const float *grad_intensity, const 
 
 /* propagator.reconstruct(phase, amplitude) for one frame per channel
  * (utils/propagator.py:276-290): phase, amplitude [C,pH,pW] float32 ->
- * intensity [D,C,pH,pW] float32 over the whole padded plane. */
+ * intensity [D,C,pH,pW] float32 over the whole padded plane, and (if out_field is not
+ * NULL) the complex plane fields [D,C,pH,pW] complex64 that the backward needs. */
 int cvqpg_blasm_reconstruct(const float *phase, const float *amplitude, int channels,
                             int pad_height, int pad_width, int planes, const double *wavelengths,
                             const double *distances, double pixel_pitch, double aperture_radius,
-                            int grid, void *ws, size_t ws_bytes, float *intensity, void *stream);
+                            int grid, void *ws, size_t ws_bytes, float *out_field,
+                            float *intensity, void *stream);
+
+/* Adjoint of cvqpg_blasm_reconstruct (the autograd backward of propagator.reconstruct with
+ * no_grad=False, utils/propagator.py:224-290, as train_2d_gaussian_rgb.py:94 calls it):
+ * grad_intensity [D,C,pH,pW] and the forward's out_field -> grad_phase and
+ * grad_amplitude [C,pH,pW] float32 (overwritten). */
+int cvqpg_blasm_reconstruct_backward(const float *phase, const float *amplitude,
+                                     const float *grad_intensity, const float *out_field,
+                                     int channels, int pad_height, int pad_width, int planes,
+                                     const double *wavelengths, const double *distances,
+                                     double pixel_pitch, double aperture_radius, int grid,
+                                     void *ws, size_t ws_bytes, float *grad_phase,
+                                     float *grad_amplitude, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Multi-plane loss.  Replaces the PyTorch loss assembly of                  */
@@ -262,6 +290,28 @@ int cvqpg_loss_forward_backward_stats(const float *intensity, const float *targe
                                       size_t ws_bytes, float *grad_intensity, float *loss,
                                       void *stream);
 
+/* The reference's loss objects as stand-alone calls (the fused step above runs all
+ * planes and both terms in one pass):
+ * cvqpg_plane_l2 replaces multiplane_loss_odak.__call__(image, target, plane_id,
+ * inject_noise, noise_ratio) (utils/propagator.py:417-449): n elements of image, target
+ * and mask (mask = masks[plane_id]); noise (NULL: inject_noise False) is the standard
+ * normal draw, scaled in-kernel by noise_ratio * (max target - min target); weights
+ * (w0, w1, w2) of MSE(I, T), MSE(I M, T~ M), MSE(I T~, T~ T~).  Writes loss[0] (device)
+ * and, if grad_image is not NULL, dL/dimage [n].  Workspace: cvqpg_plane_l2_workspace_size
+ * (a few KB, independent of n). */
+size_t cvqpg_plane_l2_workspace_size(int64_t n);
+int cvqpg_plane_l2(const float *image, const float *target, const float *mask,
+                   const float *noise, float noise_ratio, float w0, float w1, float w2,
+                   int64_t n, void *ws, size_t ws_bytes, float *grad_image, float *loss,
+                   void *stream);
+/* cvqpg_ssim_term replaces GaussianLoss(pred, target) (utils/data_utils.py:22-41, with its
+ * default lambda_ssim 0.025): loss[0] = 0.025 * 0.2 * (1 - SSIM(image, target)) for
+ * image, target [C,H,W] (image clamped to [0, 1], its gradient zero outside), and
+ * grad_image [C,H,W] = dloss/dimage.  Workspace: cvqpg_loss_workspace_size(C, 1, H, W). */
+int cvqpg_ssim_term(const float *image, const float *target, int channels, int height,
+                    int width, void *ws, size_t ws_bytes, float *grad_image, float *loss,
+                    void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fused per-Gaussian prologue and its adjoint (the reference runs these in  */
 /* PyTorch: Gaussians2D.apply_activations / compute_2d_covariance_elements / */
@@ -300,13 +350,15 @@ int cvqpg_raster_backward_raw(const float *const *raw_params, int n, int channel
 /* state flat [4][total] (exp_avg, exp_avg_sq, exp_avg_diff, neg_pre_grad,   */
 /* zero-initialised), group_lr HOST [7] per tensor (entry 0 is replaced by   */
 /* the cosine schedule lr0 -> eta_min over num_itrs).  *iteration (device)   */
-/* is the 0-based iteration and is incremented at the end.                   */
+/* is the 0-based iteration and is incremented at the end.  skip_flag        */
+/* (device, may be NULL): when *skip_flag != 0 nothing is updated and the    */
+/* iteration is not advanced (pass cvqpg_raster_overflow_flags' pointer).    */
 /* ------------------------------------------------------------------------ */
 int cvqpg_adan_step(float *const *params, const int64_t *sizes, const float *grads, float *state,
                     const float *sumsq_partials, int n_partials, const float *group_lr,
                     float lr0, float eta_min, int num_itrs, float beta1, float beta2,
                     float beta3, float eps, float max_norm, int curv_warmup, int curv_enabled,
-                    int32_t *iteration, void *stream);
+                    int32_t *iteration, const uint32_t *skip_flag, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Export and evaluation.                                                   */

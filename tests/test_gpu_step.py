@@ -182,3 +182,38 @@ def test_stress_gradients_match_central_fd(pkg):
         print(f"{n}: gpu {got} fd {fd} (max err / max |fd| = {err:.1e})", flush=True)
         assert err <= 2e-3, (n, err)
     print(f"worst relative deviation {worst:.1e}", flush=True)
+
+
+def test_key_overflow_in_a_replayed_graph_skips_the_update(pkg):
+    """Key buffers sized below the scene's key count: every replay of the captured step
+    renders nothing and must leave parameters, Adan state and iteration unchanged, with the
+    overflow counted; grow() then re-sizes and the next step equals a fresh eager step."""
+    wl, tr, raw0 = build(SPEC, seed=4)
+    P = tr.last_pairs
+    assert tr.check_capacity() and tr.overflowed_steps == 0
+    tr._alloc_raster(P // 2)
+    tr.capture()
+    before = [p.detach().clone() for p in tr.params]
+    state0 = tr.state.clone()
+    for _ in range(3):
+        tr.run()
+    torch.cuda.synchronize()
+    for a, b in zip(tr.params, before):
+        assert torch.equal(a, b)
+    assert torch.equal(tr.state, state0)
+    assert int(tr.iteration.item()) == 0
+    assert not tr.check_capacity()
+    assert tr.overflowed_steps == 3 and tr.last_pairs == P
+    print(f"P={P}, capacity {tr.capacity}: {tr.overflowed_steps} overflowed steps skipped",
+          flush=True)
+    tr.grow()
+    assert tr.capacity >= P and tr.check_capacity()
+    tr.run()
+    torch.cuda.synchronize()
+    assert tr.check_capacity() and int(tr.iteration.item()) == 1
+    _, fresh, _ = build(SPEC, seed=4)
+    fresh.step()
+    torch.cuda.synchronize()
+    assert float(tr.loss.item()) == float(fresh.loss.item())
+    for a, b in zip(tr.params, fresh.params):
+        assert torch.equal(a, b)

@@ -41,7 +41,7 @@ def test_python_binding_covers_header(pkg):
     from cvqpg_amd import _lib
     assert set(declared_symbols()) == set(_lib.SIGNATURES)
     L = _lib.lib()
-    assert L.cvqpg_abi_version() == 1
+    assert L.cvqpg_abi_version() == 2
 
 
 def test_workspace_size_and_argument_errors(pkg):
@@ -119,3 +119,26 @@ def test_batch_table_layout_and_errors(pkg):
     assert L.cvqpg_render_forward_batch(None, 3, 256, 1, None) == 1
     assert L.cvqpg_render_backward_batch(buf, 2, 256, 1, None) == 1
     assert L.cvqpg_render_forward_batch(buf, 3, 256, 0, None) == 1
+
+
+def test_overflow_flags_point_inside_the_workspace(pkg):
+    """cvqpg_raster_overflow_flags is host arithmetic only (no device access), so it can be
+    checked with a made-up workspace address."""
+    import ctypes
+    from cvqpg_amd import _lib
+    L = _lib.lib()
+    n, cap = 5000, 70000
+    nb = L.cvqpg_raster_workspace_size(n, 3, 640, 480, cap)
+    base = 1 << 40
+    out = ctypes.c_void_p()
+    assert L.cvqpg_raster_overflow_flags(ctypes.c_void_p(base), nb, n, cap, ctypes.byref(out)) == 0
+    off = out.value - base
+    assert 0 < off and off + 8 <= nb and off % 4 == 0
+    # the flags precede every capacity-dependent region: a larger capacity keeps them in place
+    nb2 = L.cvqpg_raster_workspace_size(n, 3, 640, 480, 2 * cap)
+    out2 = ctypes.c_void_p()
+    assert L.cvqpg_raster_overflow_flags(ctypes.c_void_p(base), nb2, n, 2 * cap,
+                                         ctypes.byref(out2)) == 0
+    assert out2.value == out.value
+    assert L.cvqpg_raster_overflow_flags(ctypes.c_void_p(base), nb - 1, n, cap,
+                                         ctypes.byref(out)) == 1
