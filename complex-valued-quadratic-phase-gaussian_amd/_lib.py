@@ -59,6 +59,9 @@ SIGNATURES = {
     "cvqpg_loss_forward_backward": (_c_int, [_p, _p, _p, _p, _p, ctypes.c_uint64, _p, _c_int,
                                              _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),
     "cvqpg_loss_noise": (_c_int, [ctypes.c_uint64, _p] + [_c_int] * 4 + [_p, _p]),
+    "cvqpg_loss_noalloc_max_bytes": (_c_size, []),
+    "cvqpg_loss_forward_backward_noalloc": (_c_int, [_p, _p, _p, _p, _p, _p, ctypes.c_uint64, _p,
+                                                     _c_int, _c_int, _c_int, _c_int, _p, _p, _p]),
     "cvqpg_loss_target_stats_size": (_c_size, [_c_int] * 4),
     "cvqpg_loss_target_stats": (_c_int, [_p] + [_c_int] * 4 + [_p, _c_size, _p]),
     "cvqpg_loss_forward_backward_stats": (_c_int, [_p, _p, _p, _p, _p, _p, ctypes.c_uint64, _p,

@@ -23,6 +23,8 @@
 // coefficient maps, the transposed filters and the L2 terms in a single pass over the
 // intensities, target and mask; block partial sums give the loss value without a host
 // read-back (k_loss_total).
+#include <mutex>
+
 #include "common.h"
 #include "cvqpg.h"
 
@@ -643,4 +645,57 @@ extern "C" int cvqpg_ssim_term(const float* image, const float* target, int chan
                      P, loss);
   CVQPG_LAUNCH_CHECK();
   return 0;
+}
+
+// ---------------------------------------------------------------------------------
+// The same loss with its scratch (the per-workgroup partial sums) in a static device
+// array instead of a caller's workspace, for callers that cannot allocate.  Every call on
+// one GPU shares that GPU's array; each launch writes every partial it later reads, so
+// nothing needs clearing between calls, but calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------------
+namespace cvqpg {
+constexpr size_t LOSS_SCRATCH_BYTES = 256 * 1024;
+static __device__ __align__(256) unsigned char scratch[LOSS_SCRATCH_BYTES];
+
+static std::mutex scratch_mu;
+static void* scratch_ptr[64] = {nullptr};
+
+// device address of this device's scratch (looked up once per device, with it current)
+static int loss_scratch(void** out) {
+  int dev = 0;
+  CVQPG_HIP(hipGetDevice(&dev));
+  CVQPG_CHECK_ARG(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    CVQPG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return 0;
+}
+}  // namespace cvqpg
+
+extern "C" size_t cvqpg_loss_noalloc_max_bytes(void) { return LOSS_SCRATCH_BYTES; }
+
+extern "C" int cvqpg_loss_forward_backward_noalloc(
+    const float* intensity, const float* target, const float* target_stats, const float* mask,
+    const float* noise, const float* noise_scale, uint64_t seed, const int32_t* step_counter,
+    int channels, int planes, int height, int width, float* grad_intensity, float* loss,
+    void* stream) {
+  CVQPG_CHECK_ARG(channels >= 1 && channels <= 3, "channels must be 1..3");
+  CVQPG_CHECK_ARG(planes >= 1 && planes <= 16, "planes must be 1..16");
+  CVQPG_CHECK_ARG(height >= WIN && width >= WIN, "image smaller than the SSIM window");
+  const size_t need = carve_loss(nullptr, channels, planes, height, width).bytes;
+  CVQPG_CHECK_ARG(need <= LOSS_SCRATCH_BYTES,
+                  "%d planes x %d channels of %dx%d need %zu bytes of loss scratch; the "
+                  "no-allocation entry point holds %zu (use cvqpg_loss_forward_backward_stats "
+                  "with a workspace)",
+                  planes, channels, width, height, need, LOSS_SCRATCH_BYTES);
+  void* ws = nullptr;
+  if (int rc = loss_scratch(&ws)) return rc;
+  return cvqpg_loss_forward_backward_stats(intensity, target, target_stats, mask, noise,
+                                           noise_scale, seed, step_counter, channels, planes,
+                                           height, width, ws, LOSS_SCRATCH_BYTES, grad_intensity,
+                                           loss, stream);
 }

@@ -279,6 +279,23 @@ int cvqpg_loss_forward_backward(const float *intensity, const float *target, con
 int cvqpg_loss_noise(uint64_t seed, const int32_t *step_counter, int channels, int planes,
                      int height, int width, float *noise, void *stream);
 
+/* No counterpart in the reference: the loss of cvqpg_loss_forward_backward_stats (same
+ * kernels, same results bit for bit) for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It takes no
+ * workspace: its scratch (the per-workgroup partial sums) is a static device array of
+ * cvqpg_loss_noalloc_max_bytes() bytes (256 KiB) per GPU, held until the process ends;
+ * sizes whose scratch exceeds it are rejected (return 1).  It calls no hipMalloc/hipFree.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race.  Each launch writes every partial it reads, so nothing
+ * carries over from one call to the next. */
+size_t cvqpg_loss_noalloc_max_bytes(void);
+int cvqpg_loss_forward_backward_noalloc(const float *intensity, const float *target,
+                                        const float *target_stats, const float *mask,
+                                        const float *noise, const float *noise_scale,
+                                        uint64_t seed, const int32_t *step_counter,
+                                        int channels, int planes, int height, int width,
+                                        float *grad_intensity, float *loss, void *stream);
+
 size_t cvqpg_loss_target_stats_size(int channels, int planes, int height, int width);
 int cvqpg_loss_target_stats(const float *target, int channels, int planes, int height,
                             int width, float *stats, size_t stats_bytes, void *stream);
