@@ -953,6 +953,130 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
   }
 }
 
+// A2 for the compile-time plans of one column per workgroup (CG = 1), with the next
+// plane's column prefetched into LDS while the current plane is transformed.  The plane's
+// oh rows of one column are 8-byte pieces 8 nv bytes apart; loaded into registers they
+// would have to be waited for before each plane's transform.  Here they are copied with
+// direct-to-LDS loads (global_load_lds_dword: lane i of a wave writes word i after the
+// wave's LDS base; real and imaginary parts go to two arrays), issued right after the
+// plane's first stage has consumed the previous copy, and waited for only at the next
+// plane's first barrier; the stages in between synchronise with LDS-only barriers.  The
+// LDS that makes room for the copy comes from the last forward stage, whose twiddles are
+// formed from the two-level table instead of read from a stage table.  Every LDS array
+// is a separate static object, so the compiler's wait analysis keeps the copy apart from
+// the transform's buffer.  grid (8 * per, C)
+template <class P, bool BAND>
+__global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_pf(const float2* __restrict__ Z,
+                                                              Optics o, fft::Plan plan,
+                                                              const float2* __restrict__ tabs,
+                                                              int per, float2* __restrict__ G) {
+  constexpr int N = P::n, NT = P::nt, CG = P::cg;
+  static_assert(CG == 1, "the prefetch path runs one column per workgroup");
+  using Fw = typename P::Fwd;
+  using Iv = typename P::Inv;
+  using F1 = typename Fw::template G1<CG, NT>;
+  using FL = typename Fw::template GL<CG, NT>;
+  using IL = typename Iv::template GL<CG, NT>;
+  constexpr int HALF = N / 2;                  // the crop holds at most N / 2 rows
+  constexpr int TM = Fw::toff(Fw::L - 1);      // stage tables of the middle stages
+  constexpr int NTW = (N + fft::TW_LO - 1) / fft::TW_LO + fft::TW_LO;
+  __shared__ float2 buf[P::lds];
+  __shared__ float2 stws[NTW];
+  __shared__ float2 tt[TM];
+  __shared__ float pfs[2 * HALF];              // real parts [HALF], imaginary parts [HALF]
+  const int col0 = xcd_block(blockIdx.x, per);
+  if (col0 >= o.nv) return;
+  const int c = blockIdx.y;
+  const float2* stw = stage_twiddles(plan, stws);
+  for (int i = threadIdx.x; i < TM; i += NT) tt[i] = tabs[i];
+  // rows i < oh of plane d's column: wave w of round i0 covers rows i0 + 64 w + lane
+  auto prefetch = [&](int d) {
+    const float* src =
+        reinterpret_cast<const float*>(Z + ((size_t)d * o.C + c) * o.oh * o.nv + col0);
+    for (int i0 = 0; i0 < o.oh; i0 += NT) {
+      const int i = i0 + (int)threadIdx.x;
+      const int wb = __builtin_amdgcn_readfirstlane(i0 + ((int)threadIdx.x & ~63));
+      if (i < o.oh) {
+        const float* g = src + 2 * (size_t)((unsigned)(i * o.nv));
+        __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(pfs + wb),
+                                         4, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            g + 1, (__attribute__((address_space(3))) void*)(pfs + HALF + wb), 4, 0, 0);
+      }
+    }
+  };
+  float2 A[FL::MAXB][Fw::LAST];
+  float Pg[FL::MAXB][Fw::LAST];
+#pragma unroll
+  for (int b = 0; b < FL::MAXB; ++b) {
+    int j, col;
+    FL::item(b, j, col);
+#pragma unroll
+    for (int k = 0; k < Fw::LAST; ++k) {
+      A[b][k] = make_float2(0.f, 0.f);
+      Pg[b][k] = FL::live(b) ? tf_point(o, c, j + k * FL::CNT, col0 + col) : -1.0f;
+    }
+  }
+  prefetch(0);
+  for (int d = 0; d < o.D; ++d) {
+    __syncthreads();   // plane d is in pfs (the wait covers the copy); buf is free
+    {
+      float2 v[F1::MAXB][Fw::FIRST];
+#pragma unroll
+      for (int b = 0; b < F1::MAXB; ++b) {
+        if (!F1::live(b)) continue;
+        int j, col;
+        F1::item(b, j, col);
+#pragma unroll
+        for (int k = 0; k < Fw::FIRST; ++k) {
+          float2 u = make_float2(0.f, 0.f);
+          if (in_mid_half<Fw>(k)) {
+            const int rr = j + k * F1::CNT - o.or0;
+            if ((unsigned)rr < (unsigned)o.oh) u = make_float2(pfs[rr], pfs[HALF + rr]);
+          }
+          v[b][k] = u;
+        }
+      }
+      Fw::template first<1, CG, NT>(buf, v);
+    }
+    fft::lds_barrier();   // every read of pfs is complete: it takes the next plane
+    if (d + 1 < o.D) prefetch(d + 1);
+    Fw::template middle<1, CG, NT, true>(buf, stw, plan.nh, tt);
+    float2 v[FL::MAXB][Fw::LAST];
+    Fw::template last<1, CG, NT, false>(buf, stw, plan.nh, v);
+#pragma unroll
+    for (int b = 0; b < FL::MAXB; ++b) {
+      if (!FL::live(b)) continue;
+      int j, col;
+      FL::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k) {
+        const float2 h = tf_value<BAND>(o, Pg[b][k], c, d, j + k * FL::CNT, col0 + col);
+        A[b][k].x += v[b][k].x * h.x + v[b][k].y * h.y;   // v * conj(h)
+        A[b][k].y += v[b][k].y * h.x - v[b][k].x * h.y;
+      }
+    }
+  }
+  fft::lds_barrier();
+  Iv::template first<-1, CG, NT>(buf, A);
+  fft::lds_barrier();
+  Iv::template middle<-1, CG, NT>(buf, stw, plan.nh);
+  float2 u[IL::MAXB][Iv::LAST];
+  Iv::template last<-1, CG, NT>(buf, stw, plan.nh, u);
+  float2* dst = G + (size_t)c * o.H * o.nv + col0;
+#pragma unroll
+  for (int b = 0; b < IL::MAXB; ++b) {
+    if (!IL::live(b)) continue;
+    int j, col;
+    IL::item(b, j, col);
+#pragma unroll
+    for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
+      const int rr = j + k * IL::CNT - o.gr0;
+      if ((unsigned)rr < (unsigned)o.H) dst[(unsigned)(rr * o.nv + col)] = u[b][k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------------
@@ -1179,7 +1303,12 @@ static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& p
   const size_t sh = (size_t)(col_lds_head<P>() + tab) * sizeof(float2);
   const dim3 grid(8 * per, o.C);
   const float2* tabs = w.tw_cs;
-  if constexpr (ADJ) {
+  if constexpr (ADJ && P::cg == 1) {   // all LDS of the prefetching kernel is static
+    if (o.band_active)
+      launch_fx<P>(k_a2_cols_pf<P, true>, grid, 0, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
+    else
+      launch_fx<P>(k_a2_cols_pf<P, false>, grid, 0, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
+  } else if constexpr (ADJ) {
     if (o.band_active)
       launch_fx<P>(k_a2_cols_fx<P, true>, grid, sh, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
     else

@@ -502,14 +502,24 @@ __device__ __forceinline__ void swrite(float2* buf, const float2 (&v)[Geo<N, R, 
   }
 }
 
+// Workgroup barrier for LDS exchanges only: the wave waits for its own LDS operations
+// (lgkmcnt 0) and not for its global memory operations, so global loads and stores issued
+// earlier (a prefetch into LDS, the previous plane's stores) stay in flight across it.
+// s_waitcnt immediate (gfx9 layout): vmcnt 63 (bits 3:0 and 15:14), expcnt 7 (6:4),
+// lgkmcnt 0 (11:8).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+}
+
 // One LDS-to-LDS stage; all reads finish before the barrier, all writes before the next.
 template <int R, int NS, int N, int DIR, int CG, int NT, bool TAB, class LI, class LO>
 __device__ __forceinline__ void cstage(float2* buf, const float2* tw, int nh, const float2* tt) {
   float2 v[Geo<N, R, CG, NT>::MAXB][R];
   sread<R, NS, N, DIR, CG, NT, TAB, LI>(buf, tw, nh, tt, v);
-  __syncthreads();
+  lds_barrier();
   swrite<R, NS, N, CG, NT, LO>(buf, v);
-  __syncthreads();
+  lds_barrier();
 }
 
 template <int... Rs>
