@@ -7,12 +7,14 @@ glue -> BLASM -> multi-plane L2 + SSIM loss -> full adjoint -> Adan.
 
 Prints one JSON line (rank 0).  ``time_to_psnr`` is a reduced learned-warm-start study
 (8 training pairs from 400-iteration 1080p fits, 800 render-loss fine-tuning steps, 2
-held-out 1080p scenes; ``--workload warmstart`` runs the full one):
-speed-up of the warm start to the plain fit's final per-plane PSNR.  ``roofline_blasm`` gives
-the achieved HBM rate of the FFT passes.  ``roofline`` describes the dominant kernel, timed
-with HIP events on its launch stream inside the timed region.  ``cpu_baseline``
-times the CPU oracle on a bounded sample of the same workload (see
-``cpu_baseline``).
+held-out 1080p scenes; ``--workload warmstart`` runs the full one): speed-up of the warm
+start to the plain fit's final per-plane PSNR.  ``rooflines`` has one entry per hot kernel
+(raster adjoint and forward, the two BLASM column passes and the two per-plane row passes,
+the loss), each timed with HIP events on the replay stream inside the timed steps (its stage
+is captured as a graph of its own), with its algorithmic rate against the published peak
+and its measured HBM bytes; ``roofline`` is the entry of the longest kernel.
+``cpu_baseline`` times the reference's PyTorch path on the host CPU on a bounded sample of
+the same workload (see ``cpu_baseline``).
 """
 import argparse
 import json
@@ -50,10 +52,8 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "render_bwd_traffic_rgb1080p_150k.txt")
-
-
-BLASM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blasm_traffic_rgb1080p_150k.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hot_kernels_traffic_rgb1080p_150k.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "hot_kernels_valu_rgb1080p_150k.txt")
 
 
 def read_kv(path):
@@ -68,92 +68,199 @@ def read_kv(path):
     return vals
 
 
-def blasm_roofline(stages, workload):
-    """HBM roofline of the BLASM passes (forward f1-f3 + adjoint a3-a1) from the eager
-    per-stage HIP-event times: the measured bytes of the committed rocprofv3 summary and
-    the algorithmic model of the unpruned schedule (30 two-dimensional FFTs of the 2x grid
-    at 32 B per point), both per step."""
-    vals = read_kv(BLASM_TRAFFIC_FILE)
-    if workload != WORKLOAD or "blasm_bytes_per_step" not in vals or not stages:
-        return None
-    ms = stages.get("blasm_fwd", 0.0) + stages.get("blasm_bwd", 0.0)
-    if ms <= 0:
-        return None
-    meas = float(vals["blasm_bytes_per_step"])
-    model = float(vals["blasm_model_bytes_per_step"])
-    # achieved: the bytes the kernels move (rocprofv3) over their time; model_gbs: the
-    # unpruned schedule's algorithmic bytes over the same time (above the HBM peak once the
-    # pruning removes more than the time the passes lose to latency)
-    return {"kernels": "k_f1/f2/f3 + k_a3/a2/a1 (BLASM forward and adjoint)", "bound": "hbm",
-            "achieved": meas / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": meas / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": meas, "ms": ms,
-            "model_bytes": model, "model_gbs": model / (ms * 1e-3) / 1e9,
-            "traffic_source": os.path.relpath(BLASM_TRAFFIC_FILE, ROOT)}
+# Hot kernels of the step: (stage of FusedTrainer(fine=True), kernel, bound).  Their
+# algorithmic work per launch follows the unit definitions of the hot-path measurement
+# plan:
+#   raster: unit = reference pixel-pair = keys x 256; per pair (45 + 14 C) FP32 VALU ops +
+#           (1 + 2 C) transcendentals in the adjoint, (20 + 8 C) + (1 + 2 C) in the forward
+#           (all pairs of the reference's tile lists, including those the kernels skip);
+#   BLASM:  unit = grid point of a one-dimensional pass over the 2x grid, 16 B (8 B read +
+#           8 B written, complex64); the column pass of a direction does 1 + D transforms
+#           per channel, the per-plane row pass D per channel;
+#   loss:   16 B per pixel and plane (read intensity, target, mask; write the gradient).
+HOT = (("render_bwd", "k_render_bwd", "valu"), ("blasm_bwd_cols", "k_a2_cols", "hbm"),
+       ("blasm_fwd_cols", "k_f2_cols", "hbm"), ("render_fwd", "k_render_fwd", "valu"),
+       ("loss", "k_loss_stream", "hbm"), ("blasm_bwd_rows", "k_a3_rows", "hbm"),
+       ("blasm_fwd_out", "k_f3_rows", "hbm"))
 
 
-def measured_traffic():
-    """HBM bytes per k_render_bwd launch on the default workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter values scaled by the same
-    counters' reading on a plain copy kernel of known size); None if absent."""
+def raster_ops_per_pair(C, adjoint):
+    """(VALU ops, transcendentals) per reference pixel-pair."""
+    return (45 + 14 * C, 1 + 2 * C) if adjoint else (20 + 8 * C, 1 + 2 * C)
+
+
+def kernel_model(stage, spec, pairs):
+    """Algorithmic work of one launch of a hot kernel: (amount, unit of the rate)."""
+    W, H = spec["W"], spec["H"]
+    C = len(spec["wavelengths"])
+    sys.path.insert(0, ROOT)
+    from cvqpg_amd import synthetic
+    D = len(synthetic.distances_for(spec))
+    pH, pW = synthetic.pad_size_for(W, H)
+    grid = 4 * pH * pW
+    if stage in ("render_bwd", "render_fwd"):
+        v, t = raster_ops_per_pair(C, stage == "render_bwd")
+        return pairs * (v + t), "TOP/s"
+    if stage in ("blasm_bwd_cols", "blasm_fwd_cols"):
+        return 16.0 * grid * C * (1 + D), "GB/s"
+    if stage in ("blasm_bwd_rows", "blasm_fwd_out"):
+        return 16.0 * grid * C * D, "GB/s"
+    if stage == "loss":
+        return 16.0 * D * C * H * W, "GB/s"
+    raise KeyError(stage)
+
+
+def rooflines(kernel_ms, spec, pairs, workload):
+    """One entry per timed hot kernel (average of HIP events bracketing its own graph on
+    the replay stream, inside the timed steps), with its algorithmic rate against the
+    published peak, the HBM bytes it moves per launch (committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary) and, for the VALU-bound raster, the share of cycles the SIMDs spend
+    issuing VALU instructions (committed SQ counter summary)."""
+    traffic = read_kv(TRAFFIC_FILE) if workload == WORKLOAD else {}
+    valu = read_kv(VALU_FILE) if workload == WORKLOAD else {}
+    out = []
+    for stage, kern, bound in HOT:
+        ms = kernel_ms.get(stage)
+        if not ms:
+            continue
+        amount, unit = kernel_model(stage, spec, pairs)
+        if unit == "TOP/s":
+            achieved = amount / (ms * 1e-3) / 1e12
+            peak = FP32_PEAK_TFLOPS
+        else:
+            achieved = amount / (ms * 1e-3) / 1e9
+            peak = HBM_PEAK_GBS
+        e = {"kernel": kern, "stage": stage, "bound": bound, "achieved": achieved,
+             "peak": peak, "unit": unit, "frac": achieved / peak, "avg_ms": ms,
+             "algorithmic": amount,
+             "traffic": float(traffic[kern]) if kern in traffic else None}
+        if e["traffic"]:
+            e["traffic_gbs"] = e["traffic"] / (ms * 1e-3) / 1e9
+            e["traffic_frac"] = e["traffic_gbs"] / HBM_PEAK_GBS
+        if kern + ".valu_busy" in valu:
+            e["valu_busy"] = float(valu[kern + ".valu_busy"])
+        if unit == "TOP/s":
+            e["pairs"] = pairs
+            e["ops_per_pair"] = raster_ops_per_pair(len(spec["wavelengths"]),
+                                                    stage == "render_bwd")
+        out.append(e)
+    return out
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the host's CPUs, capped by OMP_NUM_THREADS (a
+    GPU box exports its share of the host there)."""
+    n = os.cpu_count() or 1
     try:
-        return float(read_kv(TRAFFIC_FILE)["hbm_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
-
-
-def raster_flops_per_pair(C):
-    """Algorithmic FP32 operations per (pixel, Gaussian) pair of the reference math
-    (cuda/forward.cu:131-151, cuda/backward.cu:101-148), counting each add, mul,
-    compare and transcendental as one operation and an FMA as two."""
-    fwd = 17 + 9 * C          # offsets 2, mahal 9, clamp 2, exp 1, alpha 1, base 1 ; per channel 9
-    bwd = 46 + 19 * C
-    return fwd, bwd
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
 
 
 def cpu_baseline(spec, seed=0):
-    """CPU oracle on a bounded sample of the workload, scaled to step/s.
+    """The reference's PyTorch path (3-sigma / 64-px tile renderer, torch.fft BLASM, torch
+    loss + SSIM, Adan; the structure of train_2d_gaussian_rgb.py:187-211) timed on the host
+    CPU, on a bounded sample of the workload, scaled to step/s.
 
-    Sample: (a) the reference's PyTorch tile renderer (3 sigma, 64-px tiles,
-    oracle.model.render_tile_torch) forward + autograd backward in FP32 on a
-    512 x 512 window holding the Gaussians whose centres fall inside it, scaled by
-    the frame/window area ratio; (b) one forward + one inverse complex64 2D FFT of
-    the full 2816 x 4480 grid (torch.fft / pocketfft), scaled by the 4 D C FFTs of
-    the reference step (forward and autograd backward).  Elementwise work of the
-    step is not counted, so the baseline is optimistic for the CPU."""
+    A whole 1920 x 1080 step of that path keeps every tile batch for autograd (more than
+    45 GB of host memory), so the sample times, at full size, one complete instance of each
+    part of the step and scales it by its multiplicity:
+      (a) raster forward + autograd backward of the tile renderer on a 256 x 256 window (the
+          Gaussians that reach it), x frame area / window area;
+      (b) guard-band pad + glue + BLASM of one (plane, channel) on the full 2816 x 4480 grid,
+          forward + autograd backward, x planes x channels;
+      (c) one plane's 3-term L2 + SSIM loss at 1920 x 1080 x 3, forward + backward, x planes;
+      (d) the Adan step over all Gaussians, x 1.
+    The whole step of the small configuration (256 x 256 gray, 10k Gaussians) is timed as
+    well (median of 3 after one warm-up), as a check of the sampling."""
     import torch
 
     sys.path.insert(0, ROOT)
     from oracle import model as omodel
     import __graft_entry__ as ge
-    pkg = ge.load_pkg()
-    from cvqpg_amd import synthetic as syn
+    ge.load_pkg()
+    from cvqpg_amd import loss as ploss, model as pmodel, synthetic as syn, train
+    from cvqpg_amd.optimizer import Adan
+    from cvqpg_amd.propagator import propagator
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
     W, H, N = spec["W"], spec["H"], spec["N"]
     C = len(spec["wavelengths"])
+    dist = syn.distances_for(spec)
+    D = len(dist)
+    pH, pW = syn.pad_size_for(W, H)
+    parts = {}
+    # (a) raster window
     raw = syn.make_scene(N, W, H, spec["wavelengths"], spec["pitch"], spec["kappa"], seed)
-    win = 512
+    win, x0, y0, m = 256, 832, 412, 24
     tr = {k: torch.as_tensor(v) for k, v in raw.items()}
     mt = torch.tanh(tr["means_2d"])
     cx, cy = (mt[:, 0] + 1) * 0.5 * W, (mt[:, 1] + 1) * 0.5 * H
-    sel = (cx < win) & (cy < win)
+    sel = (cx > x0 - m) & (cx < x0 + win + m) & (cy > y0 - m) & (cy < y0 + win + m)
     sub = {k: v[sel].clone().requires_grad_() for k, v in tr.items()}
-    threads = torch.get_num_threads()
     t0 = time.perf_counter()
     ri = omodel.raster_inputs(sub, (W, H), spec["wavelengths"], spec["pitch"])
+    ri["means"] = ri["means"] - torch.tensor([[float(x0), float(y0)]])
     out = omodel.render_tile_torch(ri, win, win)
     (out.abs() ** 2).sum().backward()
-    t_r = time.perf_counter() - t0
-    pH, pW = syn.pad_size_for(W, H)
-    u = torch.randn(2 * pH, 2 * pW, dtype=torch.complex64)
+    parts["raster"] = (time.perf_counter() - t0, W * H / (win * win))
+    # (b) one (plane, channel) of pad + glue + BLASM
+    g = torch.Generator().manual_seed(seed)
+    h = torch.complex(torch.randn(1, H, W, generator=g), torch.randn(1, H, W, generator=g))
+    h = (0.3 * h).requires_grad_()
+    prop = propagator(resolution=(pH, pW), wavelengths=spec["wavelengths"][:1],
+                      pixel_pitch=spec["pitch"], number_of_frames=1, distances=dist[:1],
+                      aperture_size=10 * max(W, H), backend="torch")
     t0 = time.perf_counter()
-    torch.fft.ifft2(torch.fft.fft2(u))
-    t_fft = time.perf_counter() - t0
-    D = len(syn.distances_for(spec))
-    t_step = t_r * (W * H) / (win * win) + t_fft * 2 * D * C
+    padded = pmodel.soft_window_pad(h, (pH, pW))
+    ph = torch.angle(padded) % (2 * torch.pi)
+    amp = torch.clamp(torch.abs(padded), 0.0, 1.0)
+    inten = prop.reconstruct(ph, amplitude=amp, no_grad=False)[0, 0, 0]
+    crop = inten[pH // 2 - H // 2:pH // 2 + H // 2, pW // 2 - W // 2:pW // 2 + W // 2]
+    crop.sum().backward()
+    parts["blasm"] = (time.perf_counter() - t0, D * C)
+    # (c) one plane's loss
+    # (the loss's cost does not depend on the target's content: random target and depth,
+    # no defocus synthesis)
+    lf = ploss.multiplane_loss_odak(torch.rand(C, H, W, generator=g), torch.rand(1, H, W, generator=g),
+                                    number_of_planes=D, weights=[1.0, 1.0, 1.0, 0.0],
+                                    scheme="none", backend="torch")
+    pred = torch.rand(C, H, W, generator=g).requires_grad_()
+    t0 = time.perf_counter()
+    Lp = lf(pred, lf.targets[0], 0, inject_noise=True)
+    Lp = Lp + ploss.GaussianLoss(pred, lf.targets[0], backend="torch")
+    Lp.backward()
+    parts["loss"] = (time.perf_counter() - t0, D)
+    # (d) Adan over all Gaussians
+    params = [torch.nn.Parameter(torch.as_tensor(raw[k]).clone()) for k in raw]
+    for q in params:
+        q.grad = torch.randn_like(q)
+    opt = Adan([{"params": params, "lr": 1e-3}], lr=0, eps=1e-8, foreach=False)
+    t0 = time.perf_counter()
+    torch.nn.utils.clip_grad_norm_(params, 1.0)
+    opt.step()
+    parts["adan"] = (time.perf_counter() - t0, 1)
+    t_step = sum(t * k for t, k in parts.values())
+    # the small configuration's whole step
+    wl0 = train.Workload("gray256_10k", device="cpu", seed=seed, renderer="torch")
+    at = train.AutogradTrainer(wl0, backend="torch")
+    at.step()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        at.step()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    sample = "; ".join(f"{k} {t:.2f}s x{m:.1f}" for k, (t, m) in parts.items())
     return {"value": 1.0 / t_step, "unit": "step/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle PyTorch tile renderer fwd+bwd FP32 on a {win}x{win} window "
-                       f"({int(sel.sum())} Gaussians) scaled x{W * H / win / win:.1f}; one "
-                       f"fft2+ifft2 pair of {2 * pH}x{2 * pW} scaled x{2 * D * C}; "
-                       f"elementwise work not counted; {t_r:.1f}s + {t_fft:.1f}s measured")}
+            "host_cpus": os.cpu_count(),
+            "sample": (f"reference PyTorch path on the host CPU, FP32, {threads} threads: one "
+                       f"full-size instance of each part of the {W}x{H} step, scaled by its "
+                       f"multiplicity ({sample}; {int(sel.sum())} Gaussians reach the raster "
+                       f"window); whole 256x256 gray 10k step: {ts[1]:.2f}s"),
+            "parts_s": {k: t for k, (t, _) in parts.items()},
+            "gray256_10k_step_s": ts[1]}
 
 
 def stage_report(trainer, steps=3):
@@ -195,12 +302,12 @@ def bench_batch(args, dev):
     for name in ("render_fwd", "render_bwd"):
         k[name] = bt._time(lambda: bt._batched(name), 10)
     pairs = sum(t.last_pairs for t in bt.trainers) * 256
-    f_fwd, f_bwd = raster_flops_per_pair(3)
-    achieved = pairs * f_bwd / (k["render_bwd"] * 1e-3) / 1e12
+    v, t = raster_ops_per_pair(3, True)
+    achieved = pairs * (v + t) / (k["render_bwd"] * 1e-3) / 1e12
     roof = {"kernel": "k_render_bwd_batch (persistent tile-queue raster adjoint, all scenes)",
-            "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": k["render_bwd"],
-            "pairs": pairs, "flop_per_pair": f_bwd, "fwd_avg_ms": k["render_fwd"]}
+            "pairs": pairs, "ops_per_pair": [v, t], "fwd_avg_ms": k["render_fwd"]}
     n = len(bt.trainers)
     out = {"metric": BATCH_METRIC, "value": n * 1e3 / ms, "unit": "scene-step/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -294,7 +401,7 @@ def main():
     # the step's peak counts the workload's resident tensors, not the temporaries of its setup
     torch.cuda.reset_peak_memory_stats(dev)
     pH, pW = synthetic.pad_size_for(W, H)
-    f_fwd, f_bwd = raster_flops_per_pair(C)
+    one_graph_ms = None   # the kernel-timing loop's step time (fused path)
     if args.path in ("autograd", "torch"):
         trainer = train.AutogradTrainer(wl, backend="torch")
         path = ("autograd: HIP raster + torch.fft BLASM + torch loss/Adan" if args.path == "autograd"
@@ -311,41 +418,52 @@ def main():
         dt = time.perf_counter() - t0
         rasterizer.timer.enabled = False
         P = rasterizer._capacity_hint.get((N, W, H), 0)
-        k_ms = rasterizer.timer.mean_ms("k_render_bwd")
-        stages = {"k_render_fwd": rasterizer.timer.mean_ms("k_render_fwd"), "k_render_bwd": k_ms}
+        kernel_ms = {"render_fwd": rasterizer.timer.mean_ms("k_render_fwd"),
+                     "render_bwd": rasterizer.timer.mean_ms("k_render_bwd")}
+        stages = kernel_ms
     else:
-        trainer = fused.FusedTrainer(wl, warmup_iters=400, seed=0)
-        path = "fused HIP step, hipGraph replay (prologue, bin, sort, raster, BLASM, loss, adjoints, Adan)"
+        trainer = fused.FusedTrainer(wl, warmup_iters=400, seed=0, fine=True)
+        path = ("fused HIP step, hipGraph replay (prologue, bin, sort, raster, BLASM, loss, "
+                "adjoints, Adan)")
         trainer.step()
         stages = stage_report(trainer)
         print(f"[bench] eager stage ms: {json.dumps(stages)}", file=sys.stderr, flush=True)
-        trainer.capture(split="render_bwd")
+        # (1) every hot kernel's stage captured as a graph of its own, so HIP events on the
+        # replay stream bracket each of them (the events between graphs cost ~2 % of the
+        # step, so this loop only measures the kernels)
+        trainer.capture(split=[h[0] for h in HOT])
         for _ in range(args.warmup):
             trainer.run()
+        ev = {}
         torch.cuda.synchronize()
-        ev = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            trainer.run(split_events=ev)
+        torch.cuda.synchronize()
+        split_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        kernel_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in ev.items()}
+        # (2) the timed steps: the step as three graphs, the longest kernel's stage alone in
+        # the middle one, bracketed by HIP events on the replay stream
+        dominant = max(kernel_ms, key=kernel_ms.get)
+        trainer.capture(split=dominant)
+        for _ in range(args.warmup):
+            trainer.run()
+        ev = {}
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             trainer.run(split_events=ev)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+        kernel_ms[dominant] = sum(a.elapsed_time(b) for a, b in ev[dominant]) / len(ev[dominant])
         if not trainer.check_capacity():
             raise RuntimeError("key capacity exceeded during the timed steps")
         P = trainer.last_pairs
+        one_graph_ms = split_ms
     ms = 1e3 * dt / args.steps
     pairs = P * 256
-    traffic = measured_traffic() if args.workload == WORKLOAD else None
-    roof = None   # the PyTorch path has no HIP raster kernel to report
-    if k_ms:
-        achieved = pairs * f_bwd / (k_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_render_bwd (raster adjoint tile kernel)",
-                "bound": "mfma",
-                "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "avg_ms": k_ms,
-                "pairs": pairs, "flop_per_pair": f_bwd}
-        if traffic is not None:
-            roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    roofs = rooflines(kernel_ms, spec, pairs, args.workload)
+    roof = max(roofs, key=lambda e: e["avg_ms"]) if roofs else None
     peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9   # the step's own peak
     if args.calib_copy > 0:
         n = args.calib_copy * (1 << 20) // 4
@@ -372,8 +490,9 @@ def main():
            "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
                       "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
-                      "peak_hbm_gb": peak_gb, "stage_ms_eager": stages},
-           "roofline": roof, "roofline_blasm": blasm_roofline(stages, args.workload),
+                      "peak_hbm_gb": peak_gb, "stage_ms_eager": stages,
+                      "ms_per_step_all_kernels_split": one_graph_ms},
+           "roofline": roof, "rooflines": roofs,
            "cpu_baseline": cpu, "time_to_psnr": warm}
     print(json.dumps(out), flush=True)
 

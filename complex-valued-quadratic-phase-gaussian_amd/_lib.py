@@ -50,6 +50,12 @@ SIGNATURES = {
                                                           _c_size, _p, _p, _p]),
     "cvqpg_blasm_backward": (_c_int, [_p, _p, _p] + [_c_int] * 6 + [_p, _p, _c_f64, _c_f64, _c_int,
                                                                     _p, _c_size, _p, _p]),
+    "cvqpg_blasm_forward_passes": (_c_int, [_c_int, _p] + [_c_int] * 6 + [_p, _p, _c_f64, _c_f64,
+                                                                         _c_int, _p, _c_size, _p,
+                                                                         _p, _p]),
+    "cvqpg_blasm_backward_passes": (_c_int, [_c_int, _p, _p, _p] + [_c_int] * 6 + [_p, _p, _c_f64,
+                                                                                 _c_f64, _c_int, _p,
+                                                                                 _c_size, _p, _p]),
     "cvqpg_blasm_reconstruct": (_c_int, [_p, _p] + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64, _c_int,
                                                                    _p, _c_size, _p, _p, _p]),
     "cvqpg_blasm_reconstruct_backward": (_c_int, [_p] * 4 + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64,

@@ -1355,27 +1355,35 @@ static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hi
   launch_cols<CG, true>(w, o, pr, s);
 }
 
+// Transform plans of both lengths; with `tables` also (re)computes their twiddle tables in
+// the workspace (the first pass of a direction does; later passes reuse them).
 static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc,
-                 hipStream_t s) {
+                 hipStream_t s, bool tables = true) {
   CVQPG_CHECK_ARG(fft::make_plan(o.nu, pr), "FFT length %d has a prime factor above 17", o.nu);
   CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "FFT length %d has a prime factor above 17", o.nv);
   pr.tw = w.tw_r;
   pc.tw = w.tw_c;
-  const int ne = std::max(fft::twiddle_entries(o.nu), fft::twiddle_entries(o.nv));
-  hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, o.nu, w.tw_c,
-                     o.nv);
+  if (tables) {
+    const int ne = std::max(fft::twiddle_entries(o.nu), fft::twiddle_entries(o.nv));
+    hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, o.nu,
+                       w.tw_c, o.nv);
+  }
   return 0;
 }
 
 static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan& pr,
                              const fft::Plan& pc, float* out_field, float* intensity,
-                             hipStream_t s) {
-  switch (o.cg) {
-    case 8: launch_f2<8>(w, o, pr, s); break;
-    case 4: launch_f2<4>(w, o, pr, s); break;
-    case 2: launch_f2<2>(w, o, pr, s); break;
-    default: launch_f2<1>(w, o, pr, s); break;
+                             hipStream_t s, int passes = 6) {
+  if (passes & 2) {
+    switch (o.cg) {
+      case 8: launch_f2<8>(w, o, pr, s); break;
+      case 4: launch_f2<4>(w, o, pr, s); break;
+      case 2: launch_f2<2>(w, o, pr, s); break;
+      default: launch_f2<1>(w, o, pr, s); break;
+    }
+    CVQPG_LAUNCH_CHECK();
   }
+  if (!(passes & 4)) return 0;
   float2* of = reinterpret_cast<float2*>(out_field);
   const bool fx = with_row_plan(o.nv, [&](auto pl) {
     using P = decltype(pl);
@@ -1398,12 +1406,14 @@ extern "C" size_t cvqpg_blasm_workspace_size(int channels, int planes, int heigh
   return carve_blasm(nullptr, channels, planes, height, pad_height, pad_width, oh).bytes;
 }
 
-extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height, int width,
-                                   int pad_height, int pad_width, int planes,
-                                   const double* wavelengths, const double* distances,
-                                   double pixel_pitch, double aperture_radius, int grid,
-                                   void* ws, size_t ws_bytes, float* out_field, float* intensity,
-                                   void* stream) {
+extern "C" int cvqpg_blasm_forward_passes(int passes, const float* field, int channels,
+                                          int height, int width, int pad_height, int pad_width,
+                                          int planes, const double* wavelengths,
+                                          const double* distances, double pixel_pitch,
+                                          double aperture_radius, int grid, void* ws,
+                                          size_t ws_bytes, float* out_field, float* intensity,
+                                          void* stream) {
+  CVQPG_CHECK_ARG(passes >= 1 && passes <= 7, "passes must be a mask of 1, 2, 4 (got %d)", passes);
   Optics o;
   if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
@@ -1414,23 +1424,39 @@ extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height,
   hipStream_t s = (hipStream_t)stream;
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
-  if (int rc = plans(o, w, pr, pc, s)) return rc;
-  const float2* fl = reinterpret_cast<const float2*>(field);
-  const bool fx = with_row_plan(o.nv, [&](auto pl) {
-    using P = decltype(pl);
-    launch_fx<P>(k_f1_rows_fx<P>, dim3(o.pH, o.C), row_fx_lds_bytes<P>(), s, fl, o, pc, w.X);
-  });
-  if (!fx) ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, fl, o, pc, w.X);
-  CVQPG_LAUNCH_CHECK();
-  return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
+  if (int rc = plans(o, w, pr, pc, s, (passes & 1) != 0)) return rc;
+  if (passes & 1) {
+    const float2* fl = reinterpret_cast<const float2*>(field);
+    const bool fx = with_row_plan(o.nv, [&](auto pl) {
+      using P = decltype(pl);
+      launch_fx<P>(k_f1_rows_fx<P>, dim3(o.pH, o.C), row_fx_lds_bytes<P>(), s, fl, o, pc, w.X);
+    });
+    if (!fx) ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, fl, o, pc, w.X);
+    CVQPG_LAUNCH_CHECK();
+  }
+  return forward_cols_rows(w, o, pr, pc, out_field, intensity, s, passes);
 }
 
-extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intensity,
-                                    const float* out_field, int channels, int height, int width,
-                                    int pad_height, int pad_width, int planes,
-                                    const double* wavelengths, const double* distances,
-                                    double pixel_pitch, double aperture_radius, int grid,
-                                    void* ws, size_t ws_bytes, float* grad_field, void* stream) {
+extern "C" int cvqpg_blasm_forward(const float* field, int channels, int height, int width,
+                                   int pad_height, int pad_width, int planes,
+                                   const double* wavelengths, const double* distances,
+                                   double pixel_pitch, double aperture_radius, int grid,
+                                   void* ws, size_t ws_bytes, float* out_field, float* intensity,
+                                   void* stream) {
+  return cvqpg_blasm_forward_passes(7, field, channels, height, width, pad_height, pad_width,
+                                    planes, wavelengths, distances, pixel_pitch,
+                                    aperture_radius, grid, ws, ws_bytes, out_field, intensity,
+                                    stream);
+}
+
+extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
+                                           const float* grad_intensity, const float* out_field,
+                                           int channels, int height, int width, int pad_height,
+                                           int pad_width, int planes, const double* wavelengths,
+                                           const double* distances, double pixel_pitch,
+                                           double aperture_radius, int grid, void* ws,
+                                           size_t ws_bytes, float* grad_field, void* stream) {
+  CVQPG_CHECK_ARG(passes >= 1 && passes <= 7, "passes must be a mask of 1, 2, 4 (got %d)", passes);
   Optics o;
   if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
@@ -1442,30 +1468,50 @@ extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intens
   hipStream_t s = (hipStream_t)stream;
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
-  if (int rc = plans(o, w, pr, pc, s)) return rc;
-  const float2* of = reinterpret_cast<const float2*>(out_field);
-  const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
-    using P = decltype(pl);
-    launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, grad_intensity, of,
-                 o, pc, w.Y);
-  });
-  if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
-  switch (o.cg) {
-    case 8: launch_a2<8>(w, o, pr, s); break;
-    case 4: launch_a2<4>(w, o, pr, s); break;
-    case 2: launch_a2<2>(w, o, pr, s); break;
-    default: launch_a2<1>(w, o, pr, s); break;
+  if (int rc = plans(o, w, pr, pc, s, (passes & 1) != 0)) return rc;
+  if (passes & 1) {
+    const float2* of = reinterpret_cast<const float2*>(out_field);
+    const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
+      using P = decltype(pl);
+      launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s,
+                   grad_intensity, of, o, pc, w.Y);
+    });
+    if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
+    CVQPG_LAUNCH_CHECK();
   }
-  const float2* fl = reinterpret_cast<const float2*>(field);
-  float2* gf = reinterpret_cast<float2*>(grad_field);
-  const bool fx1 = with_row_plan(o.nv, [&](auto pl) {
-    using P = decltype(pl);
-    launch_fx<P>(k_a1_rows_fx<P>, dim3(o.H, o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.X, fl, o,
-                 pc, gf);
-  });
-  if (!fx1) ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s, w.X, fl, o, pc, gf);
-  CVQPG_LAUNCH_CHECK();
+  if (passes & 2) {
+    switch (o.cg) {
+      case 8: launch_a2<8>(w, o, pr, s); break;
+      case 4: launch_a2<4>(w, o, pr, s); break;
+      case 2: launch_a2<2>(w, o, pr, s); break;
+      default: launch_a2<1>(w, o, pr, s); break;
+    }
+    CVQPG_LAUNCH_CHECK();
+  }
+  if (passes & 4) {
+    const float2* fl = reinterpret_cast<const float2*>(field);
+    float2* gf = reinterpret_cast<float2*>(grad_field);
+    const bool fx1 = with_row_plan(o.nv, [&](auto pl) {
+      using P = decltype(pl);
+      launch_fx<P>(k_a1_rows_fx<P>, dim3(o.H, o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.X,
+                   fl, o, pc, gf);
+    });
+    if (!fx1) ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s, w.X, fl, o, pc, gf);
+    CVQPG_LAUNCH_CHECK();
+  }
   return 0;
+}
+
+extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intensity,
+                                    const float* out_field, int channels, int height, int width,
+                                    int pad_height, int pad_width, int planes,
+                                    const double* wavelengths, const double* distances,
+                                    double pixel_pitch, double aperture_radius, int grid,
+                                    void* ws, size_t ws_bytes, float* grad_field, void* stream) {
+  return cvqpg_blasm_backward_passes(7, field, grad_intensity, out_field, channels, height, width,
+                                     pad_height, pad_width, planes, wavelengths, distances,
+                                     pixel_pitch, aperture_radius, grid, ws, ws_bytes, grad_field,
+                                     stream);
 }
 
 extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitude, int channels,

@@ -41,8 +41,12 @@ GROUP_LR = {"means_2d": 0.01, "pre_act_scales": 0.005, "pre_act_rotation": 0.001
 
 class FusedTrainer:
     def __init__(self, wl, num_itrs=2001, lr=0.01, warmup_iters=400, primitive="curv",
-                 seed=0, inject_noise=True, headroom=1.25):
+                 seed=0, inject_noise=True, headroom=1.25, fine=False):
+        """``fine`` runs the BLASM directions pass by pass (stages blasm_fwd_rows / _cols /
+        _out and blasm_bwd_rows / _cols / _in instead of blasm_fwd / blasm_bwd), so a
+        split capture can time the column passes on their own."""
         L = _lib.lib()
+        self.fine = bool(fine)
         g = wl.gaussians
         self.wl = wl
         self.dev = g.means_2d.device
@@ -182,7 +186,7 @@ class FusedTrainer:
         ns = None if self.noise_scale is None else _lib.floats(self.noise_scale)
         b1, b2, b3 = ADAN_BETAS
         fr = lambda t: _lib.ptr(torch.view_as_real(t))
-        return [
+        stages = [
             ("prologue", self._prologue),
             ("bin_count", self._count),
             ("bin_sort", lambda: _lib.check(L.cvqpg_bin_sort(
@@ -191,17 +195,31 @@ class FusedTrainer:
             ("render_fwd", lambda: _lib.check(L.cvqpg_render_forward_packed(
                 N, C, W, H, ws, nb, cap, _lib.ptr(self.point_list), _lib.ptr(self.ranges),
                 fr(self.field), S()))),
-            ("blasm_fwd", lambda: _lib.check(L.cvqpg_blasm_forward(
-                fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
-                _lib.ptr(self.inten), S()))),
+        ]
+        blasm_fwd = lambda m: lambda: _lib.check(L.cvqpg_blasm_forward_passes(
+            m, fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
+            _lib.ptr(self.inten), S()))
+        blasm_bwd = lambda m: lambda: _lib.check(L.cvqpg_blasm_backward_passes(
+            m, fr(self.field), _lib.ptr(self.grad_inten), fr(self.plane_field), *o._common(),
+            _lib.ptr(ows), ows.numel(), fr(self.grad_field), S()))
+        if self.fine:
+            stages += [("blasm_fwd_rows", blasm_fwd(1)), ("blasm_fwd_cols", blasm_fwd(2)),
+                       ("blasm_fwd_out", blasm_fwd(4))]
+        else:
+            stages.append(("blasm_fwd", blasm_fwd(7)))
+        stages += [
             ("loss", lambda: _lib.check(L.cvqpg_loss_forward_backward_stats(
                 _lib.ptr(self.inten), _lib.ptr(self.targets), _lib.ptr(self.target_stats),
                 _lib.ptr(self.masks), None, ns,
                 self.seed, _lib.ptr(self.iteration), C, self.D, H, W, _lib.ptr(self.loss_ws),
                 self.loss_ws.numel(), _lib.ptr(self.grad_inten), _lib.ptr(self.loss), S()))),
-            ("blasm_bwd", lambda: _lib.check(L.cvqpg_blasm_backward(
-                fr(self.field), _lib.ptr(self.grad_inten), fr(self.plane_field), *o._common(),
-                _lib.ptr(ows), ows.numel(), fr(self.grad_field), S()))),
+        ]
+        if self.fine:
+            stages += [("blasm_bwd_rows", blasm_bwd(1)), ("blasm_bwd_cols", blasm_bwd(2)),
+                       ("blasm_bwd_in", blasm_bwd(4))]
+        else:
+            stages.append(("blasm_bwd", blasm_bwd(7)))
+        stages += [
             ("render_bwd", lambda: _lib.check(L.cvqpg_render_backward_tiles(
                 fr(self.grad_field), _lib.ptr(self.k), N, C, W, H, ws, nb, cap,
                 _lib.ptr(self.point_list), _lib.ptr(self.ranges), S()))),
@@ -216,6 +234,7 @@ class FusedTrainer:
                 1.0, self.warmup, 1 if self.curv_mode != 0 else 0, _lib.ptr(self.iteration),
                 self.overflow_flags, S()))),
         ]
+        return stages
 
     STAGES = ("prologue", "bin_count", "bin_sort", "render_fwd", "blasm_fwd", "loss",
               "blasm_bwd", "render_bwd", "reduce", "adan")
@@ -245,53 +264,59 @@ class FusedTrainer:
 
     # ------------------------------------------------------------------ graphs
     def capture(self, split=None):
-        """Record one step into HIP graphs.  With ``split`` the step is recorded as three
-        graphs (stages before ``split``, ``split`` alone, stages after) so the split
-        stage can be bracketed by events on the replay stream."""
+        """Record one step into HIP graphs.  ``split`` (a stage name or a list of them)
+        records each named stage as a graph of its own, between graphs of the stages in
+        between, so ``run`` can bracket the named stages with events on the replay
+        stream."""
         torch.cuda.synchronize(self.dev)
+        names_split = [] if split is None else ([split] if isinstance(split, str) else list(split))
         self._split = split
         stages = self._stages()
-        names = [n for n, _ in stages]
-        if split is None:
-            groups = [stages]
-        else:
-            i = names.index(split)
-            groups = [stages[:i], stages[i:i + 1], stages[i + 1:]]
+        groups, cur = [], []
+        for name, fn in stages:
+            if name in names_split:
+                if cur:
+                    groups.append((None, cur))
+                groups.append((name, [(name, fn)]))
+                cur = []
+            else:
+                cur.append((name, fn))
+        if cur:
+            groups.append((None, cur))
+        missing = set(names_split) - {n for n, _ in groups}
+        if missing:
+            raise ValueError(f"unknown stages {sorted(missing)}")
         graphs = []
         side = torch.cuda.Stream(self.dev)
-        for grp in groups:
+        for name, grp in groups:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(side):
                 with torch.cuda.graph(g, stream=side):
                     for _, fn in grp:
                         fn()
-            graphs.append(g)
+            graphs.append((name, g))
         torch.cuda.synchronize(self.dev)
         self.graphs = graphs
-        self.graph = graphs[0] if len(graphs) == 1 else graphs
+        self.graph = graphs
         return graphs
 
     def run(self, split_events=None):
         """One step: graph replay if captured, else eager.  With a split capture and
-        ``split_events`` (a list), events bracketing the middle graph are appended."""
+        ``split_events`` (a dict), an event pair bracketing each named stage's graph is
+        appended to ``split_events[name]``."""
         if self.graph is None:
             self.step()
             return
-        gs = self.graphs
-        if len(gs) == 1:
-            gs[0].replay()
-            return
-        gs[0].replay()
-        if split_events is not None:
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            gs[1].replay()
-            b.record()
-            split_events.append((a, b))
-        else:
-            gs[1].replay()
-        gs[2].replay()
+        for name, g in self.graphs:
+            if name is not None and split_events is not None:
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+                g.replay()
+                b.record()
+                split_events.setdefault(name, []).append((a, b))
+            else:
+                g.replay()
 
     def grad_views(self):
         """Raw gradients as tensors shaped like the parameters."""

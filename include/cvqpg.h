@@ -219,6 +219,23 @@ int cvqpg_blasm_backward(const float *field, const float *grad_intensity, const 
                          double pixel_pitch, double aperture_radius, int grid, void *ws,
                          size_t ws_bytes, float *grad_field, void *stream);
 
+/* The two calls above one pass at a time, for callers that time or schedule the passes
+ * separately: `passes` is a mask of 1 (first row pass: F1 / A3, which also computes the
+ * twiddle tables in ws), 2 (column pass: F2 / A2) and 4 (last row pass: F3 / A1).  Passes
+ * run in that order; a call without pass 1 uses the tables a pass-1 call of the same
+ * sizes left in ws.  cvqpg_blasm_forward / _backward are passes = 7. */
+int cvqpg_blasm_forward_passes(int passes, const float *field, int channels, int height,
+                               int width, int pad_height, int pad_width, int planes,
+                               const double *wavelengths, const double *distances,
+                               double pixel_pitch, double aperture_radius, int grid, void *ws,
+                               size_t ws_bytes, float *out_field, float *intensity, void *stream);
+int cvqpg_blasm_backward_passes(int passes, const float *field, const float *grad_intensity,
+                                const float *out_field, int channels, int height, int width,
+                                int pad_height, int pad_width, int planes,
+                                const double *wavelengths, const double *distances,
+                                double pixel_pitch, double aperture_radius, int grid, void *ws,
+                                size_t ws_bytes, float *grad_field, void *stream);
+
 /* propagator.reconstruct(phase, amplitude) for one frame per channel
  * (utils/propagator.py:276-290): phase, amplitude [C,pH,pW] float32 ->
  * intensity [D,C,pH,pW] float32 over the whole padded plane, and (if out_field is not
