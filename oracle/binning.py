@@ -117,3 +117,20 @@ def brute_force_tile_lists(means, radii, width, height):
         sel = (r > 0) & (lox <= tx) & (tx < hix) & (loy <= ty) & (ty < hiy)
         out.append(np.flatnonzero(sel))
     return out
+
+
+def sub_bins(bins, tiles):
+    """The key list restricted to the given tiles (ascending tile ids): the same per-tile
+    Gaussian lists, re-based so that they cover [0, P') in tile order.  Lets a caller
+    evaluate a large frame a few tile rows at a time."""
+    rng = bins["ranges"]
+    pl = bins["point_list"]
+    parts = [pl[rng[t, 0]:rng[t, 1]] for t in tiles]
+    out_pl = np.concatenate(parts).astype(np.int32) if parts else np.zeros(0, np.int32)
+    ranges = np.zeros_like(rng)
+    off = 0
+    for t, p in zip(tiles, parts):
+        if len(p):
+            ranges[t] = (off, off + len(p))
+            off += len(p)
+    return dict(P=off, point_list=out_pl if off else np.zeros(1, np.int32), ranges=ranges)

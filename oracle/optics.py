@@ -41,14 +41,19 @@ def freq_grid(n, dx, grid="odak"):
     raise ValueError(grid)
 
 
-def transfer_function(nu, nv, dx, wavelength, distance, grid="odak"):
-    """Centred BLASM transfer function [nu, nv] complex128 (odak convention)."""
-    fx = freq_grid(nu, dx, grid)           # varies along rows (dim 0) -> named FY by odak
-    fy = freq_grid(nv, dx, grid)           # varies along cols (dim 1) -> named FX by odak
+def transfer_function(nu, nv, dx, wavelength, distance, grid="odak", phase_dtype=np.float64):
+    """Centred BLASM transfer function [nu, nv] complex128 (odak convention).
+    ``phase_dtype=np.float32`` evaluates the frequency grid and the phase
+    2 pi z sqrt(1/lambda^2 - FX^2 - FY^2) in FP32, as odak does on float32 tensors (the
+    reference's precision, not the oracle's)."""
+    fx = freq_grid(nu, dx, grid).astype(phase_dtype)   # rows (dim 0) -> named FY by odak
+    fy = freq_grid(nv, dx, grid).astype(phase_dtype)   # cols (dim 1) -> named FX by odak
     FY, FX = np.meshgrid(fx, fy, indexing="ij")
-    rad = 1.0 / wavelength ** 2 - (FX ** 2 + FY ** 2)
+    one = phase_dtype(1.0)
+    rad = one / phase_dtype(wavelength) ** 2 - (FX ** 2 + FY ** 2)
     ok = rad >= 0
-    phase = 2 * np.pi * np.sqrt(np.where(ok, rad, 0.0)) * distance
+    phase = (phase_dtype(2 * np.pi) * np.sqrt(np.where(ok, rad, phase_dtype(0.0)))
+             * phase_dtype(distance)).astype(np.float64)
     x = dx * float(nu)
     y = dx * float(nv)
     fx_max = 1.0 / math.sqrt((2 * distance * (1.0 / x)) ** 2 + 1) / wavelength
@@ -91,7 +96,7 @@ class BLASM:
     """
 
     def __init__(self, pad_size, wavelengths, pixel_pitch, distances, aperture_size=None,
-                 grid="odak"):
+                 grid="odak", phase_dtype=np.float64):
         self.pH, self.pW = pad_size
         self.wavelengths = list(wavelengths)
         self.dx = float(pixel_pitch)
@@ -101,7 +106,7 @@ class BLASM:
             aperture_size = max(self.pH, self.pW)
         self.aperture = torch.as_tensor(circular_mask(nu, nv, aperture_size))
         self.H = torch.stack([torch.stack([
-            torch.as_tensor(transfer_function(nu, nv, self.dx, wl, z, grid))
+            torch.as_tensor(transfer_function(nu, nv, self.dx, wl, z, grid, phase_dtype))
             for wl in self.wavelengths]) for z in self.distances])           # [D, C, nu, nv]
 
     def field_at(self, u, d, c):

@@ -74,3 +74,26 @@ def test_float32_rounding_of_rect_upper_bound():
     v = ((pos + np.float32(r)) + np.float32(16)) - np.float32(1)
     lo, lo_y, hi, hi_y = binning.get_rect(np.array([[pos, pos]], np.float32), np.array([r]), 2048, 2048)
     assert hi[0] == int(np.trunc(v / np.float32(16)))
+
+
+def test_sub_bins_chunks_reproduce_the_whole_frame():
+    """Rendering a frame a few tile rows at a time from sub_bins gives the whole render."""
+    import torch
+    from oracle import raster
+    from oracle.step import render_binned_diff
+    from test_oracle_raster import make_inputs
+    W, H = 72, 56
+    d = make_inputs(300, W, H, 3, 9)
+    full, bins = raster.render_binned(d["means"], d["inv_cov"], d["colours"], d["phase"],
+                                      d["opac"], d["curv"], d["area"], d["radii"], d["k"],
+                                      d["dxy2"], W, H)
+    ri = {k: torch.as_tensor(d[n]).double() for k, n in
+          (("means", "means"), ("inv_cov", "inv_cov"), ("colours", "colours"),
+           ("phase", "phase"), ("opac", "opac"), ("curv", "curv"), ("area", "area"))}
+    ri["k"], ri["dxy2"] = torch.as_tensor(d["k"]).double(), d["dxy2"]
+    gx, gy = binning.tile_grid(W, H)
+    acc = torch.zeros_like(full)
+    for ty0 in range(0, gy, 2):
+        tiles = [ty * gx + tx for ty in range(ty0, min(gy, ty0 + 2)) for tx in range(gx)]
+        acc += render_binned_diff(ri, W, H, binning.sub_bins(bins, tiles))
+    assert float((acc - full).abs().max()) <= 1e-12
