@@ -6,7 +6,7 @@ with a finite Fourier aperture (export_poh.py:185-190).
 Bars: padded field relative L2 <= 1e-6; DPAC levels equal except where the FP64 level
 value lies within 1e-3 of a rounding boundary, and then within one level; PSNR within
 1e-3 dB and SSIM within 1e-5 of FP64; the 8-bit POH reconstruction through the aperture
-within 1e-4 relative L2 of FP64 BLASM on the same levels.
+within 5e-6 relative L2 of FP64 BLASM on the same levels (measured 4.8e-7).
 """
 import math
 
@@ -87,7 +87,7 @@ def test_poh_reconstruction_through_aperture(pkg):
     want = ooptics.BLASM((pH, pW), wl, pitch, dists, ap, "odak").reconstruct(u)
     e = float((got.cpu().double() - want).norm() / want.norm())
     print(f"8-bit POH through aperture {ap}: rel L2 {e:.1e}", flush=True)
-    assert e <= 1e-4
+    assert e <= 5e-6
 
 
 def test_export_poh_end_to_end(pkg):

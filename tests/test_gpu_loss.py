@@ -2,8 +2,9 @@
 
 Oracle: oracle.loss.plane_loss + oracle.loss.ssim_term per plane
 (utils/propagator.py:417-449, utils/data_utils.py:22-41, train_2d_gaussian_rgb.py:191-197),
-differentiated with torch autograd in FP64.  Bars: loss relative error <= 1e-5 and
-gradient relative L2 <= 1e-4 (FP32 kernels; SSIM moments are FP32 as in the reference).
+differentiated with torch autograd in FP64.  Bars: loss relative error <= 1e-6 and
+gradient relative L2 <= 1e-6 (FP32 kernels; SSIM moments are FP32 as in the reference;
+measured 2-7e-8 on both).
 """
 import numpy as np
 import pytest
@@ -85,8 +86,8 @@ def test_loss_and_gradient_match_fp64(pkg, C, D, H, W):
     eg = float((gg - go).norm() / go.norm())
     print(f"case C={C} D={D} {H}x{W}: loss {lg:.6f} vs {lo:.6f} (rel {el:.1e}), grad rel L2 {eg:.1e}",
           flush=True)
-    assert el <= 1e-5
-    assert eg <= 1e-4
+    assert el <= 1e-6
+    assert eg <= 1e-6
 
 
 def test_in_kernel_noise_statistics(pkg):

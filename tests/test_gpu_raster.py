@@ -2,9 +2,9 @@
 
 Bars:
   * key lists and tile ranges: bit-exact against oracle.binning (cuda/forward.cu:8-60);
-  * field: relative L2 <= 1e-4 against the FP64 dense sum (cuda/rasterizer.py:74-90),
+  * field: relative L2 <= 3e-6 against the FP64 dense sum (cuda/rasterizer.py:74-90),
     and also stated against the FP64 binned sum;
-  * adjoint: relative L2 <= 1e-4 per gradient tensor against the FP64 closed-form
+  * adjoint: relative L2 <= 2e-5 per gradient tensor against the FP64 closed-form
     adjoint (cuda/backward.cu:101-148), itself pinned to autograd and finite
     differences in tests/test_oracle_raster.py.
 """
@@ -84,9 +84,9 @@ def test_forward_matches_fp64(pkg, n, W, H, C, seed):
         if n * W * H <= 4e7 else None
     print(f"case n={n} W={W} H={H} C={C}: rel L2 vs binned FP64 {e_bin:.2e}, vs dense {e_dense}",
           flush=True)
-    assert e_bin <= 1e-4
+    assert e_bin <= 3e-6
     if e_dense is not None:
-        assert e_dense <= 1e-4
+        assert e_dense <= 3e-6
 
 
 # dense tiles (739 to 1262 keys per tile): several 256-key evaluation rounds of the adjoint
@@ -111,7 +111,7 @@ def test_backward_matches_closed_form(pkg, n, W, H, C, seed):
     print(f"case n={n} W={W} H={H} C={C}: " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()),
           flush=True)
     for k, e in errs.items():
-        assert e <= 1e-4, (k, e)
+        assert e <= 2e-5, (k, e)
 
 
 @pytest.mark.parametrize("n,W,H,C,seed", [(600, 90, 70, 3, 21), (400, 64, 64, 1, 22)])
@@ -135,9 +135,9 @@ def test_anisotropic_forward_and_backward(pkg, n, W, H, C, seed):
     errs = {k: rel(leaves[k].grad, r) for k, r in zip(NAMES, ref)}
     print(f"case n={n} W={W} H={H} C={C}: field {e_f:.1e} " +
           " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
-    assert e_f <= 1e-4
+    assert e_f <= 2e-5        # measured 0.6-2.1e-6: terms below 2^-24 are skipped
     for k, e in errs.items():
-        assert e <= 1e-4, (k, e)
+        assert e <= 2e-5, (k, e)
 
 
 def test_deterministic(pkg):
@@ -227,7 +227,7 @@ def test_binning_bit_exact_three_passes(pkg):
 
 def test_field_full_size_window_matches_fp64(pkg):
     """The 1920x1080 / 150k field, rendered whole on the GPU, on a 512 x 512 window against
-    the FP64 binned sum (cuda/forward.cu:62-162) of the same key list: <= 1e-4 relative L2."""
+    the FP64 binned sum (cuda/forward.cu:62-162) of the same key list: <= 3e-6 relative L2."""
     from cvqpg_amd import qpf_render
     d, W, H = workload_inputs("rgb1080p_150k")
     g = to_dev(d)
@@ -241,4 +241,4 @@ def test_field_full_size_window_matches_fp64(pkg):
     e = rel(out[:, y0:y0 + h, x0:x0 + w].detach(), ref)
     print(f"1080p/150k field, window {w}x{h} at ({x0},{y0}): rel L2 vs FP64 binned {e:.2e}",
           flush=True)
-    assert e <= 1e-5
+    assert e <= 3e-6

@@ -5,9 +5,9 @@ binned raster, soft_window_pad, glue, BLASM (FP64 transfer-function phase), crop
 3-term L2 + SSIM (train_2d_gaussian_rgb.py:187-205) differentiated by torch autograd in
 FP64 -- followed by oracle.adan (utils/optimizer.py:105-262, train:58-86,205-211).
 
-Bars: loss relative error <= 1e-4; raw-parameter gradients relative L2 <= 1e-3 per
-tensor (the end-to-end gradient chains the FP32 raster, two FFT passes each way and the
-FP32 SSIM moments; the per-stage bars of 1e-4 are in test_gpu_raster/blasm/loss);
+Bars (about ten times the errors measured on the MI355X): loss relative error <= 2e-6
+(measured 1-3e-7); raw-parameter gradients relative L2 <= 3e-5 per tensor (measured
+1-3e-6); central differences <= 3e-4 of the largest quotient (measured <= 2.8e-5);
 Adan: the kernel's update from the step's own gradients equals the oracle's Adan update
 from the same gradients (every element within 2 FP32 ulp of the parameter plus 1e-5 of
 the step, over four steps spanning the curvature unfreeze).  The captured HIP graph
@@ -57,9 +57,9 @@ def test_fused_step_matches_oracle(pkg, spec):
     errs = {n: float((grads_gpu[n] - grads[n]).norm() / grads[n].norm()) for n in grads}
     print(f"loss {loss_gpu:.6f} vs {float(L):.6f} (rel {el:.1e}); grads "
           + " ".join(f"{n}={e:.1e}" for n, e in errs.items()), flush=True)
-    assert el <= 1e-4
+    assert el <= 2e-6
     for n, e in errs.items():
-        assert e <= 1e-3, (n, e)
+        assert e <= 3e-5, (n, e)
 
 
 def test_adan_kernel_matches_oracle(pkg):
@@ -180,7 +180,7 @@ def test_stress_gradients_match_central_fd(pkg):
         err = np.abs(got - fd).max() / scale
         worst = max(worst, err)
         print(f"{n}: gpu {got} fd {fd} (max err / max |fd| = {err:.1e})", flush=True)
-        assert err <= 2e-3, (n, err)
+        assert err <= 3e-4, (n, err)
     print(f"worst relative deviation {worst:.1e}", flush=True)
 
 
