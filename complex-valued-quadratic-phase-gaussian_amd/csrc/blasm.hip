@@ -79,8 +79,12 @@ struct Optics {
 };
 
 struct BlasmWS {
-  float2* tw_r;   // two-level twiddle table of the column length nu
-  float2* tw_c;   // two-level twiddle table of the row length nv
+  float2* tw_r;   // two-level twiddle table of the column transform's Stockham length
+  float2* tw_c;   // two-level twiddle table of the row transform's Stockham length
+  float2* ch_r;   // Bluestein chirp and kernel spectrum of the column length (nu, m_r
+  float2* bh_r;   //   entries; empty when nu factors into the supported radices)
+  float2* ch_c;   // the same for the row length nv
+  float2* bh_c;
   float2* tw_cs;  // stage tables of a compile-time column plan (8 nu entries)
   float2* X;      // [C][pH][nv]   (forward pass 1 output; adjoint pass 2 output [C][H][nv])
   float2* Y;      // [D][C][oh][nv] (forward pass 2 output; adjoint pass 3 output)
@@ -91,8 +95,13 @@ static BlasmWS carve_blasm(void* base, int C, int D, int H, int pH, int pW, int 
   Carver cv(base);
   BlasmWS w;
   const size_t nu = 2 * (size_t)pH, nv = 2 * (size_t)pW;
-  w.tw_r = cv.take<float2>(fft::twiddle_entries((int)nu));
-  w.tw_c = cv.take<float2>(fft::twiddle_entries((int)nv));
+  const int mr = fft::stockham_len((int)nu), mc = fft::stockham_len((int)nv);
+  w.tw_r = cv.take<float2>(fft::twiddle_entries(mr));
+  w.tw_c = cv.take<float2>(fft::twiddle_entries(mc));
+  w.ch_r = cv.take<float2>(mr != (int)nu ? nu : 0);
+  w.bh_r = cv.take<float2>(mr != (int)nu ? mr : 0);
+  w.ch_c = cv.take<float2>(mc != (int)nv ? nv : 0);
+  w.bh_c = cv.take<float2>(mc != (int)nv ? mc : 0);
   w.tw_cs = cv.take<float2>(8 * nu);
   const size_t xr = (size_t)(pH > H ? pH : H);
   w.X = cv.take<float2>((size_t)C * xr * nv);
@@ -117,6 +126,44 @@ __global__ void k_twiddles(float2* __restrict__ tw_r, int nu, float2* __restrict
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.y == 0) twiddle_entry(tw_r, nu, i);
   else twiddle_entry(tw_c, nv, i);
+}
+
+// Bluestein tables of length n with Stockham length m (see fft.h), in FP64:
+//   chirp[k] = exp(-i pi (k^2 mod 2n) / n),  k < n,
+//   bhat[k]  = (1/m) sum_{|j| < n} exp(+i pi j^2 / n) exp(-2 pi i jk / m)
+//            = (1/m) (1 + 2 sum_{0 < j < n} exp(+i pi j^2 / n) cos(2 pi jk / m)),  k < m.
+// One workgroup per k (the sum over j is split over its threads).  grid m, 256 threads
+__global__ __launch_bounds__(256) void k_bluestein_tables(int n, int m, float2* __restrict__ chirp,
+                                                          float2* __restrict__ bhat) {
+  __shared__ double sr[256], si[256];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const long two_n = 2L * n;
+  double ar = 0.0, ai = 0.0;
+  for (int j = 1 + t; j < n; j += 256) {
+    double sn, cs;
+    sincospi((double)(((long)j * j) % two_n) / (double)n, &sn, &cs);
+    const double w = cospi(2.0 * (double)(((long)j * k) % m) / (double)m);
+    ar += cs * w;
+    ai += sn * w;
+  }
+  sr[t] = ar;
+  si[t] = ai;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) {
+      sr[t] += sr[t + off];
+      si[t] += si[t + off];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    bhat[k] = make_float2((float)((1.0 + 2.0 * sr[0]) / m), (float)(2.0 * si[0] / m));
+    if (k < n) {
+      double sn, cs;
+      sincospi((double)(((long)k * k) % two_n) / (double)n, &sn, &cs);
+      chirp[k] = make_float2((float)cs, (float)-sn);
+    }
+  }
 }
 
 // copy the plan's twiddle table into LDS; the caller's next barrier publishes it
@@ -202,7 +249,7 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows(const float2* __restrict__ f
                                                     Optics o, fft::Plan plan,
                                                     float2* __restrict__ X) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int r = blockIdx.x, c = blockIdx.y;
   const int sr = min(max(r - o.pad_h, 0), o.H - 1);
   const float wy = taper(r, o.pH, o.pad_h, o.H);
@@ -231,7 +278,7 @@ __global__ __launch_bounds__(ROW_NT) void k_f1_rows_polar(const float* __restric
                                                           Optics o, fft::Plan plan,
                                                           float2* __restrict__ X) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int r = blockIdx.x, c = blockIdx.y;
   const size_t base = ((size_t)c * o.pH + r) * o.pW;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) {
@@ -257,7 +304,7 @@ __global__ __launch_bounds__(ROW_NT) void k_f3_rows(const float2* __restrict__ Y
                                                     fft::Plan plan, float2* __restrict__ out,
                                                     float* __restrict__ inten) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int r = blockIdx.x, dc = blockIdx.y;
   const float2* src = Y + ((size_t)dc * o.oh + r) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
@@ -280,7 +327,7 @@ __global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ gr
                                                     const float2* __restrict__ out, Optics o,
                                                     fft::Plan plan, float2* __restrict__ Z) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int r = blockIdx.x, dc = blockIdx.y;
   const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
   const float2 cr = o.carrier[dc / o.C][dc % o.C];
@@ -309,7 +356,7 @@ __global__ __launch_bounds__(ROW_NT) void k_a1_rows(const float2* __restrict__ G
                                                     fft::Plan plan,
                                                     float2* __restrict__ grad_field) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int y = blockIdx.x, c = blockIdx.y;
   const float2* src = G + ((size_t)c * o.H + y) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
@@ -333,7 +380,7 @@ __global__ __launch_bounds__(ROW_NT) void k_a1_rows_polar(const float2* __restri
                                                           float* __restrict__ grad_phase,
                                                           float* __restrict__ grad_amp) {
   extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(o.nv));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len));
   const int y = blockIdx.x, c = blockIdx.y;
   const float2* src = G + ((size_t)c * o.H + y) * o.nv;
   for (int j = threadIdx.x; j < o.nv; j += ROW_NT) buf[fft::lidx(j)] = src[j];
@@ -368,7 +415,7 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
   const int c = blockIdx.y;
   const int n = o.nu;
   const int total = n * CG;
-  const float2* stw = stage_twiddles(plan_fwd, buf + fft::lds_elems(total));
+  const float2* stw = stage_twiddles(plan_fwd, buf + fft::lds_elems(plan_fwd.len * CG));
   for (int e = threadIdx.x; e < total; e += COL_NT) {
     const int i = e / CG, g = e % CG;
     const int r = i - o.zr0;
@@ -416,7 +463,7 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
   if (col0 >= o.nv) return;
   const int c = blockIdx.y;
   const int total = o.nu * CG;
-  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(total));
+  const float2* stw = stage_twiddles(plan, buf + fft::lds_elems(plan.len * CG));
   float2 A[COL_PER];
   float P[COL_PER];
 #pragma unroll
@@ -1084,7 +1131,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_pf(const float2* __
 // class (<= 4608, <= 8704) so the per-thread butterfly registers match the transform.
 #define ROW_LAUNCH(K, PLAN, GRID, SH, STREAM, ...)                                          \
   do {                                                                                      \
-    const int n_ = (PLAN).n;                                                                \
+    const int n_ = (PLAN).len;                                                              \
     const bool small_ = n_ <= ROW_LCAP_SMALL;                                               \
     if (fft::max_radix(PLAN) <= 8) {                                                        \
       if (small_) launch_row(K<fft::Runtime<8, ROW_LCAP_SMALL>>, GRID, SH, STREAM, __VA_ARGS__); \
@@ -1109,8 +1156,8 @@ static void launch_row(Kern k, dim3 grid, size_t sh, hipStream_t s, Args... args
   hipLaunchKernelGGL(k, grid, dim3(ROW_NT), sh, s, args...);
 }
 
-static size_t row_lds_bytes(const Optics& o) {
-  return (size_t)(fft::lds_elems(o.nv) + fft::twiddle_entries(o.nv)) * sizeof(float2);
+static size_t row_lds_bytes(const fft::Plan& pc) {
+  return (size_t)(fft::lds_elems(pc.len) + fft::twiddle_entries(pc.len)) * sizeof(float2);
 }
 
 // compile-time row plans of the standard workloads (4480: 1080p, 8320: 2160p, 1152: 256^2)
@@ -1190,7 +1237,9 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   CVQPG_CHECK_ARG(grid == 0 || grid == 1, "grid must be 0 (odak) or 1 (fft)");
   o.C = C; o.D = D; o.H = H; o.W = W; o.pH = pH; o.pW = pW;
   o.nu = 2 * pH; o.nv = 2 * pW;
-  CVQPG_CHECK_ARG(o.nv <= ROW_LCAP, "padded width %d exceeds the supported row length", pW);
+  const int mu = fft::stockham_len(o.nu), mv = fft::stockham_len(o.nv);
+  CVQPG_CHECK_ARG(mv <= ROW_LCAP, "padded width %d exceeds the supported row length "
+                  "(transform length %d, Stockham length %d > %d)", pW, o.nv, mv, ROW_LCAP);
   o.pad_h = (pH - H) / 2;
   o.pad_w = (pW - W) / 2;
   o.zr0 = pH - pH / 2;   // odak zero_pad: (2 pH)//2 - pH//2
@@ -1201,8 +1250,9 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
     o.or0 = o.nu / 4 + pH / 2 - H / 2; o.oc0 = o.nv / 4 + pW / 2 - W / 2; o.oh = H; o.ow = W;
   }
   o.gr0 = o.zr0 + o.pad_h;
-  o.cg = pick_cg(o.nu, o.nv);
-  CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length", pH);
+  o.cg = pick_cg(mu, o.nv);
+  CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length "
+                  "(transform length %d, Stockham length %d > %d)", pH, o.nu, mu, COL_LCAP);
   // the pruned transforms rely on every non-zero input and every kept output lying in
   // the centre half [n/4, 3n/4) of the 2x grid
   CVQPG_CHECK_ARG(o.zr0 == o.nu / 4 && o.zc0 == o.nv / 4 && o.or0 >= o.nu / 4 &&
@@ -1272,7 +1322,7 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
 template <int CG, class F>
 static void launch_f2_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
   const int nb = o.nv / CG, per = ceil_div(nb, 8);
-  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
+  const size_t sh = (size_t)(fft::lds_elems(pr.len * CG) + fft::twiddle_entries(pr.len)) * sizeof(float2);
   allow_lds(k_f2_cols<CG, F>, sh);
   hipLaunchKernelGGL((k_f2_cols<CG, F>), dim3(8 * per, o.C), dim3(COL_NT), sh, s, w.X, o, pr,
                      per, w.Y);
@@ -1281,7 +1331,7 @@ static void launch_f2_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, 
 template <int CG, class F>
 static void launch_a2_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
   const int nb = o.nv / CG, per = ceil_div(nb, 8);
-  const size_t sh = (size_t)(fft::lds_elems(o.nu * CG) + fft::twiddle_entries(o.nu)) * sizeof(float2);
+  const size_t sh = (size_t)(fft::lds_elems(pr.len * CG) + fft::twiddle_entries(pr.len)) * sizeof(float2);
   allow_lds(k_a2_cols<CG, F>, sh);
   hipLaunchKernelGGL((k_a2_cols<CG, F>), dim3(8 * per, o.C), dim3(COL_NT), sh, s, w.Y, o, pr,
                      per, w.X);
@@ -1359,14 +1409,28 @@ static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hi
 // the workspace (the first pass of a direction does; later passes reuse them).
 static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc,
                  hipStream_t s, bool tables = true) {
-  CVQPG_CHECK_ARG(fft::make_plan(o.nu, pr), "FFT length %d has a prime factor above 17", o.nu);
-  CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "FFT length %d has a prime factor above 17", o.nv);
+  CVQPG_CHECK_ARG(fft::make_plan(o.nu, pr), "no FFT plan for length %d", o.nu);
+  CVQPG_CHECK_ARG(fft::make_plan(o.nv, pc), "no FFT plan for length %d", o.nv);
   pr.tw = w.tw_r;
   pc.tw = w.tw_c;
+  if (fft::is_bluestein(pr)) {
+    pr.chirp = w.ch_r;
+    pr.bhat = w.bh_r;
+  }
+  if (fft::is_bluestein(pc)) {
+    pc.chirp = w.ch_c;
+    pc.bhat = w.bh_c;
+  }
   if (tables) {
-    const int ne = std::max(fft::twiddle_entries(o.nu), fft::twiddle_entries(o.nv));
-    hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, o.nu,
-                       w.tw_c, o.nv);
+    const int ne = std::max(fft::twiddle_entries(pr.len), fft::twiddle_entries(pc.len));
+    hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, pr.len,
+                       w.tw_c, pc.len);
+    if (fft::is_bluestein(pr))
+      hipLaunchKernelGGL(k_bluestein_tables, dim3(pr.len), dim3(256), 0, s, pr.n, pr.len, w.ch_r,
+                         w.bh_r);
+    if (fft::is_bluestein(pc))
+      hipLaunchKernelGGL(k_bluestein_tables, dim3(pc.len), dim3(256), 0, s, pc.n, pc.len, w.ch_c,
+                         w.bh_c);
   }
   return 0;
 }
@@ -1390,7 +1454,7 @@ static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan&
     launch_fx<P>(k_f3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.Y,
                  o, pc, of, intensity);
   });
-  if (!fx) ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, w.Y, o, pc, of, intensity);
+  if (!fx) ROW_LAUNCH(k_f3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, w.Y, o, pc, of, intensity);
   CVQPG_LAUNCH_CHECK();
   return 0;
 }
@@ -1431,7 +1495,7 @@ extern "C" int cvqpg_blasm_forward_passes(int passes, const float* field, int ch
       using P = decltype(pl);
       launch_fx<P>(k_f1_rows_fx<P>, dim3(o.pH, o.C), row_fx_lds_bytes<P>(), s, fl, o, pc, w.X);
     });
-    if (!fx) ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, fl, o, pc, w.X);
+    if (!fx) ROW_LAUNCH(k_f1_rows, pc, dim3(o.pH, o.C), row_lds_bytes(pc), s, fl, o, pc, w.X);
     CVQPG_LAUNCH_CHECK();
   }
   return forward_cols_rows(w, o, pr, pc, out_field, intensity, s, passes);
@@ -1476,7 +1540,7 @@ extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
       launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s,
                    grad_intensity, of, o, pc, w.Y);
     });
-    if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
+    if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, grad_intensity, of, o, pc, w.Y);
     CVQPG_LAUNCH_CHECK();
   }
   if (passes & 2) {
@@ -1496,7 +1560,7 @@ extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
       launch_fx<P>(k_a1_rows_fx<P>, dim3(o.H, o.C), row_fx_lds_bytes<P>(), s, (const float2*)w.X,
                    fl, o, pc, gf);
     });
-    if (!fx1) ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(o), s, w.X, fl, o, pc, gf);
+    if (!fx1) ROW_LAUNCH(k_a1_rows, pc, dim3(o.H, o.C), row_lds_bytes(pc), s, w.X, fl, o, pc, gf);
     CVQPG_LAUNCH_CHECK();
   }
   return 0;
@@ -1531,7 +1595,7 @@ extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitud
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
   if (int rc = plans(o, w, pr, pc, s)) return rc;
-  ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, phase, amplitude, o, pc, w.X);
+  ROW_LAUNCH(k_f1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(pc), s, phase, amplitude, o, pc, w.X);
   CVQPG_LAUNCH_CHECK();
   return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
 }
@@ -1561,14 +1625,14 @@ extern "C" int cvqpg_blasm_reconstruct_backward(
     launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, grad_intensity,
                  of, o, pc, w.Y);
   });
-  if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(o), s, grad_intensity, of, o, pc, w.Y);
+  if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, grad_intensity, of, o, pc, w.Y);
   switch (o.cg) {
     case 8: launch_a2<8>(w, o, pr, s); break;
     case 4: launch_a2<4>(w, o, pr, s); break;
     case 2: launch_a2<2>(w, o, pr, s); break;
     default: launch_a2<1>(w, o, pr, s); break;
   }
-  ROW_LAUNCH(k_a1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(o), s, (const float2*)w.X, phase,
+  ROW_LAUNCH(k_a1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(pc), s, (const float2*)w.X, phase,
              amplitude, o, pc, grad_phase, grad_amplitude);
   CVQPG_LAUNCH_CHECK();
   return 0;

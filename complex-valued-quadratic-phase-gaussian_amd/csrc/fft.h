@@ -16,6 +16,16 @@
 // powers of W^k.  DIR = +1 is the forward transform (e^{-i}), DIR = -1 the
 // unnormalised inverse (e^{+i}).  Radices 2, 3, 4, 5 and 8 have dedicated butterflies;
 // the odd primes 7, 11, 13 and 17 use a symmetric direct DFT with compile-time roots.
+//
+// A length n with a prime factor above 17 runs as Bluestein's chirp-z transform on a
+// Stockham transform of 7-smooth length m >= 2n - 1 held in the same LDS buffer.  With
+// c_j = e^{-i pi j^2 / n} (j^2 reduced modulo 2n exactly) and jk = (j^2 + k^2 - (k-j)^2)/2,
+//     X_k = sum_j x_j e^{-2 pi i jk/n} = c_k sum_j (x_j c_j) conj(c_{k-j}),
+// a linear convolution with the even sequence b_j = conj(c_j), |j| < n, which the m-point
+// circular convolution reproduces exactly because m >= 2n - 1:
+//     a = (x c, 0 ...),  X = c * IFFT_m(FFT_m(a) * B),  B = FFT_m(b) / m.
+// The unnormalised inverse conjugates c and B (B is even, so FFT_m(conj b) = conj(B)).
+// c (n entries) and B (m entries) are tables in global memory built in FP64 per length.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,11 +37,14 @@ namespace fft {
 constexpr int MAX_STAGES = 16;
 
 struct Plan {
-  int n;
+  int n;             // transform length
+  int len;           // Stockham length: n, or the Bluestein length m (chirp != nullptr)
   int nstages;
   int radix[MAX_STAGES];
-  int nh;            // ceil(n / 64)
-  const float2* tw;  // two-level table: tw[h] = W_n^{64 h} (h < nh), tw[nh + l] = W_n^l (l < 64)
+  int nh;            // ceil(len / 64)
+  const float2* tw;  // two-level table of len: tw[h] = W^{64 h} (h < nh), tw[nh + l] = W^l (l < 64)
+  const float2* chirp;  // Bluestein only: c_j, j < n
+  const float2* bhat;   // Bluestein only: B_k, k < m
 };
 
 constexpr int TW_LO = 64;
@@ -42,13 +55,16 @@ __host__ __device__ inline int twiddle_entries(int n) { return (n + TW_LO - 1) /
 __device__ __forceinline__ int lidx(int i) { return i + (i >> 4); }
 __host__ __device__ inline int lds_elems(int n) { return n + (n >> 4) + 1; }
 
-// Factorisation into supported radices; powers of two become radix-8 and radix-4
-// stages (never radix 2 unless n has a single factor 2); returns false if n has a
-// prime factor above 17.
-inline bool make_plan(int n, Plan& p) {
+// Factorisation of a Stockham length into supported radices; powers of two become
+// radix-8 and radix-4 stages (never radix 2 unless n has a single factor 2); returns
+// false if n has a prime factor above 17.
+inline bool make_stages(int n, Plan& p) {
   p.n = n;
+  p.len = n;
   p.nstages = 0;
   p.nh = (n + TW_LO - 1) / TW_LO;
+  p.chirp = nullptr;
+  p.bhat = nullptr;
   if (n < 2) return false;
   int m = n, a = 0;
   while (m % 2 == 0) {
@@ -79,6 +95,36 @@ inline bool make_plan(int n, Plan& p) {
   }
   return m == 1;
 }
+
+// smallest 2^a 3^b 5^c 7^d >= v
+inline int smooth7_at_least(int v) {
+  int best = 0;
+  for (long a = 1; a < 2L * v + 2; a *= 2)
+    for (long b = a; b < 2L * v + 2; b *= 3)
+      for (long c = b; c < 2L * v + 2; c *= 5)
+        for (long d = c; d < 2L * v + 2; d *= 7)
+          if (d >= v && (best == 0 || d < best)) best = (int)d;
+  return best;
+}
+
+// Stockham length of a transform of length n: n itself when it factors into the
+// supported radices, else the Bluestein length (smallest 7-smooth m >= 2n - 1).
+inline int stockham_len(int n) {
+  Plan p;
+  return make_stages(n, p) ? n : smooth7_at_least(2 * n - 1);
+}
+
+// Plan of a transform of length n >= 2 (Bluestein when n has a prime factor above 17; the
+// caller sets tw, and for Bluestein chirp and bhat, to tables of p.len / p.n).
+inline bool make_plan(int n, Plan& p) {
+  if (make_stages(n, p)) return true;
+  if (n < 2) return false;
+  if (!make_stages(smooth7_at_least(2 * n - 1), p)) return false;
+  p.n = n;
+  return true;
+}
+
+inline bool is_bluestein(const Plan& p) { return p.len != p.n; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -354,28 +400,60 @@ __device__ __forceinline__ void stage(float2* buf, int n, int ns, const float2* 
 template <int DIR, int CG, int NT, int LCAP, int MAXR = 17>
 __device__ __forceinline__ void run(float2* buf, const Plan& p, const float2* tw) {
   int ns = 1;
+  const int n = p.len;
   for (int s = 0; s < p.nstages; ++s) {
     const int R = p.radix[s];
     switch (R) {
-      case 2: stage<2, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
-      case 3: stage<3, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
-      case 4: stage<4, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
-      case 5: stage<5, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
-      case 7: stage<7, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
-      case 8: stage<8, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh); break;
+      case 2: stage<2, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
+      case 3: stage<3, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
+      case 4: stage<4, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
+      case 5: stage<5, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
+      case 7: stage<7, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
+      case 8: stage<8, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh); break;
       case 11:
-        if constexpr (MAXR >= 11) stage<11, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
+        if constexpr (MAXR >= 11) stage<11, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh);
         break;
       case 13:
-        if constexpr (MAXR >= 13) stage<13, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
+        if constexpr (MAXR >= 13) stage<13, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh);
         break;
       case 17:
-        if constexpr (MAXR >= 17) stage<17, DIR, CG, NT, LCAP>(buf, p.n, ns, tw, p.nh);
+        if constexpr (MAXR >= 17) stage<17, DIR, CG, NT, LCAP>(buf, n, ns, tw, p.nh);
         break;
       default: break;
     }
     ns *= R;
   }
+}
+
+// Bluestein transform of CG interleaved sequences of length p.n whose inputs occupy the
+// first p.n positions of buf; the Stockham stages run on p.len positions (the caller's
+// buffer holds p.len * CG elements).  Same contract as run: synchronised on entry, the
+// result in the first p.n positions and synchronised on exit.
+template <int DIR, int CG, int NT, int LCAP, int MAXR>
+__device__ __forceinline__ void bluestein(float2* buf, const Plan& p, const float2* tw) {
+  const int n = p.n, total = p.len * CG;
+  for (int e = threadIdx.x; e < total; e += NT) {
+    const int i = e / CG;
+    float2 v = make_float2(0.f, 0.f);
+    if (i < n) {
+      const float2 c = p.chirp[i];
+      v = cmul(buf[lidx(e)], DIR > 0 ? c : make_float2(c.x, -c.y));
+    }
+    buf[lidx(e)] = v;
+  }
+  __syncthreads();
+  run<1, CG, NT, LCAP, MAXR>(buf, p, tw);
+  for (int e = threadIdx.x; e < total; e += NT) {
+    const float2 b = p.bhat[e / CG];
+    buf[lidx(e)] = cmul(buf[lidx(e)], DIR > 0 ? b : make_float2(b.x, -b.y));
+  }
+  __syncthreads();
+  run<-1, CG, NT, LCAP, MAXR>(buf, p, tw);
+  for (int e = threadIdx.x; e < n * CG; e += NT) {
+    const float2 c = p.chirp[e / CG];
+    buf[lidx(e)] = cmul(buf[lidx(e)], DIR > 0 ? c : make_float2(c.x, -c.y));
+  }
+  __syncthreads();
 }
 
 inline int max_radix(const Plan& p) {
@@ -616,13 +694,15 @@ struct Pipe<N, Rad<Rs...>, Lays<Ls...>> {
   }
 };
 
-// Runtime<MAXR, LCAP>: any supported length up to LCAP / CG, stage list read from the plan.
+// Runtime<MAXR, LCAP>: any length whose Stockham length is at most LCAP / CG, stage list
+// read from the plan; Bluestein for lengths with a prime factor above 17.
 template <int MAXR, int LCAP>
 struct Runtime {
   static constexpr int n = 0;
   template <int DIR, int CG, int NT>
   __device__ __forceinline__ static void exec(float2* buf, const Plan& p, const float2* tw) {
-    run<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
+    if (p.chirp) bluestein<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
+    else run<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
   }
 };
 

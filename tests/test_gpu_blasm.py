@@ -8,9 +8,10 @@ Bars (relative L2, FP32 kernels vs FP64 oracle with an FP64 transfer-function ph
     pipeline;
   * at full 1080p size, the z = 0 plane equals |glue(field)|^2 (a size-independent
     identity of the transform pair).
-FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13 on runtime plans, and the
-compile-time plans of the three standard workloads (transform lengths 1152; 2816 x 4480;
-4992 x 8320) at their real sizes.
+FFT sizes cover radices 2/4/8, 3, 5, 7, 11 and 13 on runtime plans, Bluestein (chirp-z)
+transforms for lengths with the prime factors 19, 23, 59, 73 and 101 (up to a 2432 x 2432
+grid), and the compile-time plans of the three standard workloads (transform lengths
+1152; 2816 x 4480; 4992 x 8320) at their real sizes.
 """
 import numpy as np
 import pytest
@@ -63,7 +64,17 @@ CASES = [  # (C, H, W, pH, pW, distances, grid)
 ]
 
 
-@pytest.mark.parametrize("C,H,W,pH,pW,dists,grid", CASES)
+# Transform lengths with a prime factor above 17 run as Bluestein (chirp-z) transforms
+BLUESTEIN_CASES = [
+    (3, 40, 50, 46, 118, [0.0, 3e-3], "odak"),            # 92 = 2^2 23, 236 = 2^2 59
+    (1, 100, 60, 202, 146, [1e-3], "fft"),                # 404 = 2^2 101, 292 = 2^2 73
+    (3, 64, 928, 384, 1216, [2e-3, 5e-3], "odak"),        # rows 2432 = 2^7 19 only
+    (1, 928, 928, 1216, 1216, [0.0, 4e-3], "odak"),       # 2432 x 2432
+]
+
+
+@pytest.mark.parametrize("C,H,W,pH,pW,dists,grid", CASES + BLUESTEIN_CASES,
+                         ids=[f"{c[3]}x{c[4]}" for c in CASES + BLUESTEIN_CASES])
 def test_forward_and_adjoint_match_fp64(pkg, C, H, W, pH, pW, dists, grid):
     from cvqpg_amd.optics import HologramOptics, propagate
     wl = [639e-9, 515e-9, 473e-9][:C] if C == 3 else [515e-9]
@@ -136,6 +147,29 @@ def test_reconstruct_full_plane(pkg):
     e = rel(got, ref)
     print(f"reconstruct rel L2 {e:.1e}", flush=True)
     assert e <= 1e-5
+
+
+def test_bluestein_z0_identity_and_linearity(pkg):
+    """2432 x 2432 (Bluestein on both axes): the z = 0 plane is |glue(field)|^2, and the
+    planes' complex fields are linear in the glued input (field a + 2 i field b)."""
+    from cvqpg_amd.optics import HologramOptics
+    C, H, W, pH, pW = 1, 928, 928, 1216, 1216
+    rng = np.random.default_rng(4)
+    opt = HologramOptics((W, H), (pH, pW), [515e-9], 8e-6, [0.0, 3e-3], aperture_size=10 * W)
+    fa = (0.3 * (rng.normal(size=(C, H, W)) + 1j * rng.normal(size=(C, H, W)))).astype(np.complex64)
+    fb = (0.3 * (rng.normal(size=(C, H, W)) + 1j * rng.normal(size=(C, H, W)))).astype(np.complex64)
+    fa, fb = fa / (3 * np.abs(fa).max()), fb / (3 * np.abs(fb).max())   # |.| < 1: glue is identity
+    outs = []
+    for f in (fa, fb, fa + 2j * fb):
+        inten, field = opt.forward(torch.as_tensor(f).to(DEV), keep_field=True)
+        outs.append((inten.clone(), field.clone()))
+    torch.cuda.synchronize()
+    a = torch.as_tensor(np.abs(fa) ** 2).to(DEV)
+    e0 = float((outs[0][0][0] - a).norm() / a.norm())
+    lin = outs[0][1] + 2j * outs[1][1]
+    el = float((outs[2][1] - lin).abs().norm() / lin.abs().norm())
+    print(f"2432^2 Bluestein: z=0 identity rel L2 {e0:.1e}, linearity rel L2 {el:.1e}", flush=True)
+    assert e0 <= 1e-5 and el <= 1e-5
 
 
 def test_full_size_z0_identity(pkg):

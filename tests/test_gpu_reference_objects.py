@@ -44,7 +44,8 @@ def phases_amps(C, pH, pW, seed):
     return ph, am
 
 
-@pytest.mark.parametrize("pH,pW,zs", [(128, 192, [0.0, 2e-3]), (384, 448, [1e-3, 4e-3, 6e-3])])
+@pytest.mark.parametrize("pH,pW,zs", [(128, 192, [0.0, 2e-3]), (384, 448, [1e-3, 4e-3, 6e-3]),
+                                      (120, 146, [1e-3, 3e-3])])   # 292 = 2^2 73: Bluestein
 def test_reconstruct_hip_matches_fp64_and_torch(pkg, pH, pW, zs):
     ph, am = phases_amps(3, pH, pW, 1)
     hip = make_prop(pH, pW, zs, "hip", aperture_size=10 * max(pH, pW))

@@ -26,6 +26,9 @@ SPEC = dict(W=64, H=48, N=300, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
             distances=[0.0, 3e-3, 6e-3], kappa=0.05)
 SPEC_GRAY = dict(W=48, H=48, N=200, wavelengths=[515e-9], pitch=8e-6,
                  distances=[0.0, 2.5e-3, 5e-3], kappa=0.02)
+# a 928-px-wide frame: padded width 1216, row transforms of 2432 = 2^7 19 (Bluestein)
+SPEC_BLUE = dict(W=928, H=48, N=1200, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
+                 distances=[0.0, 3e-3], kappa=0.02)
 
 
 def build(spec, seed=0, warmup=0):
@@ -42,7 +45,7 @@ def oracle_cfg(wl):
                             ap.distances, aperture_size=ap.aperture_size, grid="odak")
 
 
-@pytest.mark.parametrize("spec", [SPEC, SPEC_GRAY], ids=["rgb", "gray"])
+@pytest.mark.parametrize("spec", [SPEC, SPEC_GRAY, SPEC_BLUE], ids=["rgb", "gray", "bluestein"])
 def test_fused_step_matches_oracle(pkg, spec):
     wl, tr, raw0 = build(spec)
     tr.step()
