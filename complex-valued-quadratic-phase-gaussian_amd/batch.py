@@ -116,9 +116,16 @@ class BatchTrainer:
         for s in self.streams:
             s.wait_stream(main)
         for i in self.order:
-            with torch.cuda.stream(self.streams[self.assign[i]]):
+            side = self.streams[self.assign[i]]
+            with torch.cuda.stream(side):
                 for n in names:
                     self._stages[i][n]()
+                    # every per-scene stage enqueues on the current stream only; a stage that
+                    # forks onto streams of its own inside these side streams is not supported
+                    # (the capture of such a nesting has crashed in hipStreamEndCapture)
+                    if torch.cuda.current_stream(self.dev) != side:
+                        raise RuntimeError(f"stage {n!r} of scene {i} changed the current "
+                                           "stream; per-scene stages must not fork streams")
         for s in self.streams:
             main.wait_stream(s)
 

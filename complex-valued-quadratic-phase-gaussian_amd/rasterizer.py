@@ -203,19 +203,30 @@ def qpf_render(means_2d, inv_cov, colours, phase, opacities, curvature, area,
 
 
 def qpf_render_reference(means_2d, inv_cov, colours, phase, opacities, curvature, area,
-                         k, dxy2, width, height):
-    """Dense all-pairs evaluation in PyTorch (no culling), the testing helper that the
-    reference exports next to qpf_render (cuda/rasterizer.py:74-90).  O(N*H*W) memory:
-    small inputs only; never used by qpf_render."""
-    dev = means_2d.device
-    C = colours.shape[1]
-    ys, xs = torch.meshgrid(torch.arange(height, device=dev, dtype=means_2d.dtype),
-                            torch.arange(width, device=dev, dtype=means_2d.dtype), indexing="ij")
-    d = means_2d[:, None, :] - torch.stack([xs, ys], -1).reshape(1, -1, 2)
-    dx, dy = d[..., 0], d[..., 1]
-    m = inv_cov[:, 0:1] * dx * dx + 2 * inv_cov[:, 1:2] * dx * dy + inv_cov[:, 2:3] * dy * dy
-    alpha = opacities[:, None] * torch.exp(torch.clamp(-0.5 * m, min=-50.0))
-    base = 0.5 * curvature[:, None] * m * area[:, None] * dxy2
-    ph = phase.t()[..., None] + k.view(C, 1, 1) * base[None]
-    amp = colours.t()[..., None] * alpha[None]
-    return torch.polar(amp, ph).sum(1).view(C, height, width)
+                         k, dxy2, width, height, pixels_per_chunk=1 << 22):
+    """Dense evaluation of every (Gaussian, pixel) term in PyTorch, without culling: the
+    testing helper the reference exports next to qpf_render (cuda/rasterizer.py:74-90).
+    Differentiable; works a band of image rows at a time so that at most
+    ``pixels_per_chunk`` (Gaussian, pixel) terms are live.  Never used by qpf_render."""
+    n = means_2d.shape[0]
+    i00, i01, i11 = inv_cov.unbind(1)
+    xs = torch.arange(width, device=means_2d.device, dtype=means_2d.dtype)
+    ex = means_2d[:, 0, None, None] - xs                    # [N, 1, W]
+    scale = (0.5 * dxy2) * curvature * area                 # per Gaussian, times the Mahalanobis
+    rows = max(1, pixels_per_chunk // max(1, n * width))
+    bands = []
+    for y0 in range(0, height, rows):
+        ys = torch.arange(y0, min(height, y0 + rows), device=means_2d.device, dtype=means_2d.dtype)
+        ey = means_2d[:, 1, None, None] - ys[:, None]       # [N, R, 1]
+        q = (i00[:, None, None] * ex * ex + 2.0 * i01[:, None, None] * ex * ey
+             + i11[:, None, None] * ey * ey)               # [N, R, W]
+        weight = opacities[:, None, None] * torch.exp(torch.clamp(-0.5 * q, min=-50.0))
+        chirp = scale[:, None, None] * q
+        per_channel = []
+        for c in range(colours.shape[1]):
+            theta = phase[:, c, None, None] + k[c] * chirp
+            amp = colours[:, c, None, None] * weight
+            per_channel.append(torch.complex((amp * torch.cos(theta)).sum(0),
+                                             (amp * torch.sin(theta)).sum(0)))
+        bands.append(torch.stack(per_channel))              # [C, R, W]
+    return torch.cat(bands, dim=1)
