@@ -76,7 +76,9 @@ def read_kv(path):
 #           (all pairs of the reference's tile lists, including those the kernels skip);
 #   BLASM:  unit = grid point of a one-dimensional pass over the 2x grid, 16 B (8 B read +
 #           8 B written, complex64); the column pass of a direction does 1 + D transforms
-#           per channel, the per-plane row pass D per channel;
+#           of every column of the 2x grid per channel, the per-plane row pass D per
+#           channel, of the rows that survive the crop to the image (the other rows of
+#           the 2x grid are cropped away before that pass, so it never transforms them);
 #   loss:   16 B per pixel and plane (read intensity, target, mask; write the gradient).
 HOT = (("render_bwd", "k_render_bwd", "valu"), ("blasm_bwd_cols", "k_a2_cols", "hbm"),
        ("blasm_fwd_cols", "k_f2_cols", "hbm"), ("render_fwd", "k_render_fwd", "valu"),
@@ -104,7 +106,7 @@ def kernel_model(stage, spec, pairs):
     if stage in ("blasm_bwd_cols", "blasm_fwd_cols"):
         return 16.0 * grid * C * (1 + D), "GB/s"
     if stage in ("blasm_bwd_rows", "blasm_fwd_out"):
-        return 16.0 * grid * C * D, "GB/s"
+        return 16.0 * H * (2 * pW) * C * D, "GB/s"
     if stage == "loss":
         return 16.0 * D * C * H * W, "GB/s"
     raise KeyError(stage)

@@ -695,14 +695,21 @@ struct Pipe<N, Rad<Rs...>, Lays<Ls...>> {
 };
 
 // Runtime<MAXR, LCAP>: any length whose Stockham length is at most LCAP / CG, stage list
-// read from the plan; Bluestein for lengths with a prime factor above 17.
+// read from the plan; Bluestein for lengths with a prime factor above 17.  A Bluestein
+// length is 7-smooth, so only the MAXR = 8 instances (which the host picks for it by the
+// plan's largest radix) carry the Bluestein code.
 template <int MAXR, int LCAP>
 struct Runtime {
   static constexpr int n = 0;
   template <int DIR, int CG, int NT>
   __device__ __forceinline__ static void exec(float2* buf, const Plan& p, const float2* tw) {
-    if (p.chirp) bluestein<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
-    else run<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
+    if constexpr (MAXR <= 8) {
+      if (p.chirp) {
+        bluestein<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
+        return;
+      }
+    }
+    run<DIR, CG, NT, LCAP, MAXR>(buf, p, tw);
   }
 };
 
