@@ -40,12 +40,26 @@ def parse():
                          "path on the GPU (3-sigma / 64-px tile renderer, torch.fft BLASM, "
                          "torch loss and Adan), the denominator of the speed-up target")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-full-step", action="store_true",
+                    help="also time one whole step of the reference's PyTorch path at full "
+                         "size on the host CPU (about a minute) and report it as the CPU "
+                         "baseline, next to the sampled estimate")
     ap.add_argument("--no-warmstart", action="store_true",
                     help="skip the reduced warm-start time-to-PSNR study of the default run")
     ap.add_argument("--workload", default=WORKLOAD,
                     help="rgb1080p_150k (the metric's workload), gray256_10k, rgb4k_600k, "
                          "batch16 (16 uneven scenes, batched vs sequential) or warmstart "
                          "(full learned warm-start time-to-PSNR study)")
+    ap.add_argument("--warm-pairs", type=int, default=16,
+                    help="--workload warmstart: training pairs (1080p plain fits)")
+    ap.add_argument("--warm-train-steps", type=int, default=600,
+                    help="--workload warmstart: supervised Adam steps on the pairs")
+    ap.add_argument("--warm-eval", type=int, default=4,
+                    help="--workload warmstart: held-out 1920x1080 scenes (2000-iteration "
+                         "plain fits); the 512x512 set keeps 4 scenes")
+    ap.add_argument("--warm-out", default=None, metavar="PATH",
+                    help="--workload warmstart: also write the whole result (every held-out "
+                         "scene's row) as JSON to PATH")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB after the timed steps (a "
                          "plain copy kernel of known size for scaling HBM byte counters)")
@@ -265,6 +279,24 @@ def cpu_baseline(spec, seed=0):
             "gray256_10k_step_s": ts[1]}
 
 
+def cpu_full_step(name, seed=0):
+    """One whole training step of the reference's PyTorch path (``--renderer torch`` with
+    ``grad_ckpt``, torch.fft BLASM, torch loss + SSIM, Adan) at the workload's full size
+    on the host CPU: the tile loop keeps one tile's output per tile for autograd instead of
+    its Gaussian batches, which bounds host memory.  Returns (seconds, threads)."""
+    import torch
+    from cvqpg_amd import train
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
+    wl = train.Workload(name, device="cpu", seed=seed, renderer="torch")
+    wl.args_prop.grad_ckpt = True
+    at = train.AutogradTrainer(wl, backend="torch")
+    print(f"[bench] CPU full step of {name} ({threads} threads)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    at.step()
+    return time.perf_counter() - t0, threads
+
+
 def stage_report(trainer, steps=3):
     """Eager per-stage GPU times (HIP events between the stages), for diagnostics."""
     import torch
@@ -360,7 +392,15 @@ def bench_warmstart(args, dev, quick=False):
                                    eval_sets=(((1920, 1080), 2, 1000),), log=log,
                                    with_mean=False)
     else:
-        res = warmstart.experiment(dev, log=log)
+        res = warmstart.experiment(dev, n_train=args.warm_pairs,
+                                   train_steps=args.warm_train_steps,
+                                   eval_sets=(((1920, 1080), args.warm_eval, 2000),
+                                              ((512, 512), 4, 1000)), log=log)
+        if args.warm_out:
+            rows = {"network": res["network"],
+                    "evals": [{k: v for k, v in e.items()} for e in res["evals"]]}
+            with open(args.warm_out, "w") as f:
+                json.dump(rows, f, indent=1, default=float)
     return warm_summary(res)
 
 
@@ -485,6 +525,15 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(spec)
+        if args.cpu_full_step:
+            t_full, threads = cpu_full_step(args.workload)
+            cpu["sampled_value"] = cpu["value"]
+            cpu["value"] = 1.0 / t_full
+            cpu["full_step_s"] = t_full
+            cpu["sample"] = (f"one whole {W}x{H} step of the reference PyTorch path on the host "
+                             f"CPU, FP32, {threads} threads, gradient checkpointing of the tile "
+                             f"loop: {t_full:.1f}s (the sampled estimate: "
+                             f"{1.0 / cpu['sampled_value']:.1f}s)")
     out = {"metric": METRIC, "value": 1e3 / ms, "unit": "step/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",

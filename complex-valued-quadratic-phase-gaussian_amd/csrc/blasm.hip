@@ -58,6 +58,15 @@ constexpr int COL_NT = CVQPG_COL_NT;
 constexpr int COL_LCAP = 6144;   // column length x columns per workgroup
 constexpr int COL_PER = COL_LCAP / COL_NT;
 
+// what a column-pass launch helper does: build the plan's stage tables, run the pass
+constexpr int COLS_TABLES = 1, COLS_RUN = 2;
+
+// whether a pass-mask call (re)builds the workspace's transform tables: pass 1 or an
+// explicit CVQPG_BLASM_TABLES does, unless CVQPG_BLASM_TABLES_KEPT says they are current
+static bool build_tables(int passes) {
+  return (passes & (1 | CVQPG_BLASM_TABLES)) && !(passes & CVQPG_BLASM_TABLES_KEPT);
+}
+
 struct Optics {
   int C, D, H, W, pH, pW, nu, nv;
   int pad_h, pad_w;      // hologram offset inside the padded field
@@ -1344,10 +1353,13 @@ static void launch_cols_k(const BlasmWS& w, const Optics& o, const fft::Plan& pr
 }
 
 template <class P, bool ADJ>
-static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s,
+                           int what) {
   static_assert(P::Fwd::TABLE + P::Inv::TABLE <= 8 * P::n, "stage tables exceed their region");
   constexpr int ne = P::Fwd::TABLE + P::Inv::TABLE;
-  hipLaunchKernelGGL(k_pipe_tables<P>, dim3(ceil_div(ne, 256)), dim3(256), 0, s, w.tw_cs);
+  if (what & COLS_TABLES)
+    hipLaunchKernelGGL(k_pipe_tables<P>, dim3(ceil_div(ne, 256)), dim3(256), 0, s, w.tw_cs);
+  if (!(what & COLS_RUN)) return;
   const int nb = o.nv / P::cg, per = ceil_div(nb, 8);
   const int tab = ADJ ? P::Fwd::TABLE : P::Inv::TABLE;
   const size_t sh = (size_t)(col_lds_head<P>() + tab) * sizeof(float2);
@@ -1372,23 +1384,25 @@ static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& p
 }
 
 template <int CG, bool ADJ>
-static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
+static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s,
+                        int what = COLS_TABLES | COLS_RUN) {
   // compile-time column plans of the standard workloads (2816: 1080p, 4992: 2160p, 1152: 256^2)
   if constexpr (CG == 2) {
     if (o.nu == 2816) {
-      if constexpr (ADJ) return launch_cols_fx<ColP2816A, ADJ>(w, o, pr, s);
-      else return launch_cols_fx<ColP2816F, ADJ>(w, o, pr, s);
+      if constexpr (ADJ) return launch_cols_fx<ColP2816A, ADJ>(w, o, pr, s, what);
+      else return launch_cols_fx<ColP2816F, ADJ>(w, o, pr, s, what);
     }
-    if (o.nu == 1664) return launch_cols_fx<ColP1664, ADJ>(w, o, pr, s);
-    if (o.nu == 2176) return launch_cols_fx<ColP2176, ADJ>(w, o, pr, s);
-    if (o.nu == 2048) return launch_cols_fx<ColP2048, ADJ>(w, o, pr, s);
+    if (o.nu == 1664) return launch_cols_fx<ColP1664, ADJ>(w, o, pr, s, what);
+    if (o.nu == 2176) return launch_cols_fx<ColP2176, ADJ>(w, o, pr, s, what);
+    if (o.nu == 2048) return launch_cols_fx<ColP2048, ADJ>(w, o, pr, s, what);
   }
   if constexpr (CG == 4) {
-    if (o.nu == 1152) return launch_cols_fx<ColP1152, ADJ>(w, o, pr, s);
+    if (o.nu == 1152) return launch_cols_fx<ColP1152, ADJ>(w, o, pr, s, what);
   }
   if constexpr (CG == 1) {
-    if (o.nu == 4992) return launch_cols_fx<ColP4992, ADJ>(w, o, pr, s);
+    if (o.nu == 4992) return launch_cols_fx<ColP4992, ADJ>(w, o, pr, s, what);
   }
+  if (!(what & COLS_RUN)) return;   // runtime plans have no stage tables
   const int mr = fft::max_radix(pr);
   if (mr <= 8) launch_cols_k<CG, ADJ, fft::Runtime<8, COL_LCAP>>(w, o, pr, s);
   else if (mr <= 13) launch_cols_k<CG, ADJ, fft::Runtime<13, COL_LCAP>>(w, o, pr, s);
@@ -1396,13 +1410,38 @@ static void launch_cols(const BlasmWS& w, const Optics& o, const fft::Plan& pr, 
 }
 
 template <int CG>
-static void launch_f2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
-  launch_cols<CG, false>(w, o, pr, s);
+static void launch_f2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s,
+                      int what = COLS_TABLES | COLS_RUN) {
+  launch_cols<CG, false>(w, o, pr, s, what);
 }
 
 template <int CG>
-static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s) {
-  launch_cols<CG, true>(w, o, pr, s);
+static void launch_a2(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s,
+                      int what = COLS_TABLES | COLS_RUN) {
+  launch_cols<CG, true>(w, o, pr, s, what);
+}
+
+// The column kernels' stage tables depend only on the plan's radices, and the forward
+// and the adjoint plan of a length share them, so one region serves both directions.
+template <class A, class B>
+constexpr bool same_pipe_radices() {
+  if (A::L != B::L || A::TABLE != B::TABLE) return false;
+  for (int i = 0; i < A::L; ++i)
+    if (A::RR[i] != B::RR[i]) return false;
+  return true;
+}
+static_assert(same_pipe_radices<ColP2816F::Fwd, ColP2816A::Fwd>() &&
+                  same_pipe_radices<ColP2816F::Inv, ColP2816A::Inv>(),
+              "the 2816-point forward and adjoint column plans must share stage tables");
+
+static void launch_cols_for(const BlasmWS& w, const Optics& o, const fft::Plan& pr, hipStream_t s,
+                            bool adj, int what) {
+  switch (o.cg) {
+    case 8: adj ? launch_a2<8>(w, o, pr, s, what) : launch_f2<8>(w, o, pr, s, what); break;
+    case 4: adj ? launch_a2<4>(w, o, pr, s, what) : launch_f2<4>(w, o, pr, s, what); break;
+    case 2: adj ? launch_a2<2>(w, o, pr, s, what) : launch_f2<2>(w, o, pr, s, what); break;
+    default: adj ? launch_a2<1>(w, o, pr, s, what) : launch_f2<1>(w, o, pr, s, what); break;
+  }
 }
 
 // Transform plans of both lengths; with `tables` also (re)computes their twiddle tables in
@@ -1422,6 +1461,7 @@ static int plans(const Optics& o, const BlasmWS& w, fft::Plan& pr, fft::Plan& pc
     pc.bhat = w.bh_c;
   }
   if (tables) {
+    launch_cols_for(w, o, pr, s, false, COLS_TABLES);
     const int ne = std::max(fft::twiddle_entries(pr.len), fft::twiddle_entries(pc.len));
     hipLaunchKernelGGL(k_twiddles, dim3(ceil_div(ne, 256), 2), dim3(256), 0, s, w.tw_r, pr.len,
                        w.tw_c, pc.len);
@@ -1439,12 +1479,7 @@ static int forward_cols_rows(const BlasmWS& w, const Optics& o, const fft::Plan&
                              const fft::Plan& pc, float* out_field, float* intensity,
                              hipStream_t s, int passes = 6) {
   if (passes & 2) {
-    switch (o.cg) {
-      case 8: launch_f2<8>(w, o, pr, s); break;
-      case 4: launch_f2<4>(w, o, pr, s); break;
-      case 2: launch_f2<2>(w, o, pr, s); break;
-      default: launch_f2<1>(w, o, pr, s); break;
-    }
+    launch_cols_for(w, o, pr, s, false, COLS_RUN);
     CVQPG_LAUNCH_CHECK();
   }
   if (!(passes & 4)) return 0;
@@ -1477,18 +1512,21 @@ extern "C" int cvqpg_blasm_forward_passes(int passes, const float* field, int ch
                                           double aperture_radius, int grid, void* ws,
                                           size_t ws_bytes, float* out_field, float* intensity,
                                           void* stream) {
-  CVQPG_CHECK_ARG(passes >= 1 && passes <= 7, "passes must be a mask of 1, 2, 4 (got %d)", passes);
+  CVQPG_CHECK_ARG(passes >= 0 && passes <= 31 && (passes & 7 || passes & CVQPG_BLASM_TABLES),
+                  "passes must be a mask of 1, 2, 4, CVQPG_BLASM_TABLES and "
+                  "CVQPG_BLASM_TABLES_KEPT (got %d)", passes);
   Optics o;
   if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
     return rc;
-  CVQPG_CHECK_ARG(field && intensity && ws, "NULL field/intensity/workspace");
+  CVQPG_CHECK_ARG((field || !(passes & 1)) && (intensity || !(passes & 4)) && ws,
+                  "NULL field/intensity/workspace");
   const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
   hipStream_t s = (hipStream_t)stream;
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
-  if (int rc = plans(o, w, pr, pc, s, (passes & 1) != 0)) return rc;
+  if (int rc = plans(o, w, pr, pc, s, build_tables(passes))) return rc;
   if (passes & 1) {
     const float2* fl = reinterpret_cast<const float2*>(field);
     const bool fx = with_row_plan(o.nv, [&](auto pl) {
@@ -1520,19 +1558,21 @@ extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
                                            const double* distances, double pixel_pitch,
                                            double aperture_radius, int grid, void* ws,
                                            size_t ws_bytes, float* grad_field, void* stream) {
-  CVQPG_CHECK_ARG(passes >= 1 && passes <= 7, "passes must be a mask of 1, 2, 4 (got %d)", passes);
+  CVQPG_CHECK_ARG(passes >= 0 && passes <= 31 && (passes & 7 || passes & CVQPG_BLASM_TABLES),
+                  "passes must be a mask of 1, 2, 4, CVQPG_BLASM_TABLES and "
+                  "CVQPG_BLASM_TABLES_KEPT (got %d)", passes);
   Optics o;
   if (int rc = make_optics(o, channels, height, width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 0))
     return rc;
-  CVQPG_CHECK_ARG(field && grad_intensity && out_field && grad_field && ws,
+  CVQPG_CHECK_ARG(((field && grad_intensity && out_field && grad_field) || !(passes & 7)) && ws,
                   "NULL argument pointer");
   const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
   hipStream_t s = (hipStream_t)stream;
   const BlasmWS w = carve_blasm(ws, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   fft::Plan pr, pc;
-  if (int rc = plans(o, w, pr, pc, s, (passes & 1) != 0)) return rc;
+  if (int rc = plans(o, w, pr, pc, s, build_tables(passes))) return rc;
   if (passes & 1) {
     const float2* of = reinterpret_cast<const float2*>(out_field);
     const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
@@ -1544,12 +1584,7 @@ extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
     CVQPG_LAUNCH_CHECK();
   }
   if (passes & 2) {
-    switch (o.cg) {
-      case 8: launch_a2<8>(w, o, pr, s); break;
-      case 4: launch_a2<4>(w, o, pr, s); break;
-      case 2: launch_a2<2>(w, o, pr, s); break;
-      default: launch_a2<1>(w, o, pr, s); break;
-    }
+    launch_cols_for(w, o, pr, s, true, COLS_RUN);
     CVQPG_LAUNCH_CHECK();
   }
   if (passes & 4) {
@@ -1626,12 +1661,7 @@ extern "C" int cvqpg_blasm_reconstruct_backward(
                  of, o, pc, w.Y);
   });
   if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, grad_intensity, of, o, pc, w.Y);
-  switch (o.cg) {
-    case 8: launch_a2<8>(w, o, pr, s); break;
-    case 4: launch_a2<4>(w, o, pr, s); break;
-    case 2: launch_a2<2>(w, o, pr, s); break;
-    default: launch_a2<1>(w, o, pr, s); break;
-  }
+  launch_cols_for(w, o, pr, s, true, COLS_RUN);
   ROW_LAUNCH(k_a1_rows_polar, pc, dim3(o.pH, o.C), row_lds_bytes(pc), s, (const float2*)w.X, phase,
              amplitude, o, pc, grad_phase, grad_amplitude);
   CVQPG_LAUNCH_CHECK();

@@ -33,6 +33,7 @@ from .optics import HologramOptics
 from .rasterizer import workspace_bytes
 
 TILE = 16
+TABLES, TABLES_KEPT = 8, 16   # CVQPG_BLASM_TABLES, CVQPG_BLASM_TABLES_KEPT
 ADAN_BETAS = (0.98, 0.92, 0.99)
 GROUP_LR = {"means_2d": 0.01, "pre_act_scales": 0.005, "pre_act_rotation": 0.001,
             "colours": 0.0025, "pre_act_phase": 0.0025, "pre_act_opacities": 0.025,
@@ -108,6 +109,11 @@ class FusedTrainer:
         self.group_lr = [GROUP_LR[n] for n in PARAM_NAMES]
         self.graph = None
         self.capacity = 0
+        # the BLASM transform tables of these sizes, once per fit (the step keeps them)
+        ows = self.optics._workspace(dev)
+        _lib.check(L.cvqpg_blasm_forward_passes(
+            TABLES, None, *self.optics._common(), _lib.ptr(ows), ows.numel(), None, None,
+            _lib.stream_handle(dev)))
         self._size_keys(headroom)
 
     # ------------------------------------------------------------------ sizing
@@ -196,11 +202,13 @@ class FusedTrainer:
                 N, C, W, H, ws, nb, cap, _lib.ptr(self.point_list), _lib.ptr(self.ranges),
                 fr(self.field), S()))),
         ]
+        # the transform tables were built once in __init__ (TABLES_KEPT: no launches for
+        # them in the step)
         blasm_fwd = lambda m: lambda: _lib.check(L.cvqpg_blasm_forward_passes(
-            m, fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
+            m | TABLES_KEPT, fr(self.field), *o._common(), _lib.ptr(ows), ows.numel(), fr(self.plane_field),
             _lib.ptr(self.inten), S()))
         blasm_bwd = lambda m: lambda: _lib.check(L.cvqpg_blasm_backward_passes(
-            m, fr(self.field), _lib.ptr(self.grad_inten), fr(self.plane_field), *o._common(),
+            m | TABLES_KEPT, fr(self.field), _lib.ptr(self.grad_inten), fr(self.plane_field), *o._common(),
             _lib.ptr(ows), ows.numel(), fr(self.grad_field), S()))
         if self.fine:
             stages += [("blasm_fwd_rows", blasm_fwd(1)), ("blasm_fwd_cols", blasm_fwd(2)),

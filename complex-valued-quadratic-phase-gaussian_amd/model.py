@@ -213,46 +213,64 @@ class Scene2D:
         return self._render_torch(ri, k, dxy2, W, H)
 
     def _render_torch(self, ri, k, dxy2, W, H):
-        """The reference's PyTorch tile loop (model_2d_gaussian_rgb.py:393-419)."""
+        """The reference's PyTorch tile loop (model_2d_gaussian_rgb.py:393-419); with
+        ``args_prop.grad_ckpt`` each tile's body runs under activation checkpointing
+        (:407-416), so autograd keeps one tile's output instead of its batches."""
+        from torch.utils.checkpoint import checkpoint
         C = len(self.wavelengths)
-        dev = self.device
         c00, c01, c11 = ri["cov"]
-        inv = ri["inv_cov"]
-        means, opac, curv, scales = ri["means"], ri["opac"], ri["curv"], ri["scales"]
         cf = (ri["colours"] * torch.exp(1j * ri["phase"])).to(torch.complex64)
         radius = 3.0 * torch.sqrt(0.5 * (c00 + c11) + torch.sqrt(((c00 - c11) * 0.5) ** 2 + c01 ** 2))
         ts = self.tile_size
         diag = math.sqrt(2 * (ts / 2) ** 2)
-        holo = torch.zeros((C, H, W), dtype=torch.complex64, device=dev)
+        holo = torch.zeros((C, H, W), dtype=torch.complex64, device=self.device)
+        scales = ri["scales"]
         area = (scales[:, 0] * scales[:, 1]).abs()
+        tensors = (ri["means"], ri["inv_cov"], ri["opac"], ri["curv"], area, cf, radius)
+        ckpt = bool(getattr(self.args_prop, "grad_ckpt", False)) and torch.is_grad_enabled()
         for y0 in range(0, H, ts):
             y1 = min(y0 + ts, H)
             for x0 in range(0, W, ts):
                 x1 = min(x0 + ts, W)
-                cx, cy = (x0 + x1) / 2.0, (y0 + y1) / 2.0
-                mask = (means[:, 0] - cx) ** 2 + (means[:, 1] - cy) ** 2 < (radius + diag) ** 2
-                if not bool(mask.any()):
-                    continue
-                ys = torch.arange(y0, y1, device=dev, dtype=torch.float32)
-                xs = torch.arange(x0, x1, device=dev, dtype=torch.float32)
-                gy, gx = torch.meshgrid(ys, xs, indexing="ij")
-                pts = torch.stack([gx, gy], -1).reshape(1, -1, 2)
-                idx = torch.nonzero(mask).squeeze(1)
-                acc = torch.zeros((C, pts.shape[1]), dtype=torch.complex64, device=dev)
-                for g0 in range(0, idx.numel(), self.gauss_batch):
-                    b = idx[g0:g0 + self.gauss_batch]
-                    diff = pts - means[b].unsqueeze(1)
-                    dx, dy = diff[..., 0], diff[..., 1]
-                    m = dx * dx * inv[b, 0, None] + 2 * dx * dy * inv[b, 1, None] + dy * dy * inv[b, 2, None]
-                    alphas = opac[b, None] * torch.exp(torch.clamp(-0.5 * m, min=-50))
-                    if self.phase_iso:
-                        base = 0.5 * curv[b, None] * (dx * dx + dy * dy) * dxy2
-                    else:
-                        base = 0.5 * curv[b, None] * m * area[b, None] * dxy2
-                    mod = torch.polar(alphas.unsqueeze(0).expand(C, -1, -1), k.view(C, 1, 1) * base.unsqueeze(0))
-                    acc = acc + torch.einsum("cbp,bc->cp", mod, cf[b])
-                holo[:, y0:y1, x0:x1] = holo[:, y0:y1, x0:x1] + acc.view(C, y1 - y0, x1 - x0)
+                if ckpt:
+                    acc = checkpoint(self._tile_torch, (x0, y0, x1, y1), k, dxy2, diag, *tensors,
+                                     use_reentrant=False)
+                else:
+                    acc = self._tile_torch((x0, y0, x1, y1), k, dxy2, diag, *tensors)
+                if acc is not None:
+                    holo[:, y0:y1, x0:x1] = holo[:, y0:y1, x0:x1] + acc.view(C, y1 - y0, x1 - x0)
         return holo
+
+    def _tile_torch(self, box, k, dxy2, diag, means, inv, opac, curv, area, cf, radius):
+        """Complex contribution [C, pixels] of the tile ``box`` = (x0, y0, x1, y1): the
+        Gaussians whose 3-sigma disc reaches the tile's circumscribed circle, evaluated
+        ``gauss_batch`` at a time; None when no Gaussian reaches the tile."""
+        x0, y0, x1, y1 = box
+        dev = self.device
+        C = len(self.wavelengths)
+        cx, cy = (x0 + x1) / 2.0, (y0 + y1) / 2.0
+        mask = (means[:, 0] - cx) ** 2 + (means[:, 1] - cy) ** 2 < (radius + diag) ** 2
+        if not bool(mask.any()):
+            return None
+        ys = torch.arange(y0, y1, device=dev, dtype=torch.float32)
+        xs = torch.arange(x0, x1, device=dev, dtype=torch.float32)
+        gy, gx = torch.meshgrid(ys, xs, indexing="ij")
+        pts = torch.stack([gx, gy], -1).reshape(1, -1, 2)
+        idx = torch.nonzero(mask).squeeze(1)
+        acc = torch.zeros((C, pts.shape[1]), dtype=torch.complex64, device=dev)
+        for g0 in range(0, idx.numel(), self.gauss_batch):
+            b = idx[g0:g0 + self.gauss_batch]
+            diff = pts - means[b].unsqueeze(1)
+            dx, dy = diff[..., 0], diff[..., 1]
+            m = dx * dx * inv[b, 0, None] + 2 * dx * dy * inv[b, 1, None] + dy * dy * inv[b, 2, None]
+            alphas = opac[b, None] * torch.exp(torch.clamp(-0.5 * m, min=-50))
+            if self.phase_iso:
+                base = 0.5 * curv[b, None] * (dx * dx + dy * dy) * dxy2
+            else:
+                base = 0.5 * curv[b, None] * m * area[b, None] * dxy2
+            mod = torch.polar(alphas.unsqueeze(0).expand(C, -1, -1), k.view(C, 1, 1) * base.unsqueeze(0))
+            acc = acc + torch.einsum("cbp,bc->cp", mod, cf[b])
+        return acc
 
     def render(self, img_size=None):
         """Padded complex hologram [C, pH, pW] (model_2d_gaussian_rgb.py:441-444)."""

@@ -221,9 +221,15 @@ int cvqpg_blasm_backward(const float *field, const float *grad_intensity, const 
 
 /* The two calls above one pass at a time, for callers that time or schedule the passes
  * separately: `passes` is a mask of 1 (first row pass: F1 / A3, which also computes the
- * twiddle tables in ws), 2 (column pass: F2 / A2) and 4 (last row pass: F3 / A1).  Passes
+ * transform tables in ws), 2 (column pass: F2 / A2) and 4 (last row pass: F3 / A1).  Passes
  * run in that order; a call without pass 1 uses the tables a pass-1 call of the same
- * sizes left in ws.  cvqpg_blasm_forward / _backward are passes = 7. */
+ * sizes left in ws.  cvqpg_blasm_forward / _backward are passes = 7.
+ * CVQPG_BLASM_TABLES in the mask builds every transform table of these sizes in ws (alone:
+ * nothing else runs and the data pointers may be NULL); CVQPG_BLASM_TABLES_KEPT says the
+ * tables in ws are current (built by an earlier call with the same sizes), so pass 1 does
+ * not rebuild them: a training loop builds them once and saves their launches per step. */
+#define CVQPG_BLASM_TABLES 8
+#define CVQPG_BLASM_TABLES_KEPT 16
 int cvqpg_blasm_forward_passes(int passes, const float *field, int channels, int height,
                                int width, int pad_height, int pad_width, int planes,
                                const double *wavelengths, const double *distances,

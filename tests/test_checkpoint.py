@@ -49,3 +49,33 @@ def test_loads_a_checkpoint_written_the_reference_way(pkg, tmp_path):
     b.load_gaussians(str(p))
     for n in PARAM_NAMES:
         assert torch.equal(getattr(a, n), getattr(b, n)), n
+
+
+def test_torch_renderer_gradient_checkpointing(pkg):
+    """--renderer torch with args_prop.grad_ckpt (model_2d_gaussian_rgb.py:407-416): the
+    field and every parameter gradient equal those of the plain tile loop, which is the
+    oracle's 3-sigma renderer on the same activations."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from oracle import model as omodel
+    from cvqpg_amd.model import PARAM_NAMES, Scene2D, make_trainable_2d
+    out, grads = [], []
+    for ckpt in (False, True):
+        g = _model(pkg, 5, n=400, size=(150, 90))
+        g.args_prop.renderer = "torch"
+        g.args_prop.grad_ckpt = ckpt
+        make_trainable_2d(g)
+        holo = Scene2D(g, g.args_prop).render_unpadded()
+        (holo.abs() ** 2).sum().backward()
+        out.append(holo.detach())
+        grads.append([getattr(g, n).grad.clone() for n in PARAM_NAMES])
+        if not ckpt:
+            raw = {n: getattr(g, n).detach() for n in PARAM_NAMES}
+            ri = omodel.raster_inputs(raw, (150, 90), g.args_prop.wavelengths,
+                                      g.args_prop.pixel_pitch)
+            ref = omodel.render_tile_torch(ri, 150, 90)
+            assert float((holo.detach() - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
+    assert torch.equal(out[0], out[1])
+    for a, b in zip(*grads):
+        assert torch.allclose(a, b, rtol=1e-6, atol=1e-7 * float(a.abs().max()))

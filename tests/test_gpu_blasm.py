@@ -189,3 +189,40 @@ def test_full_size_z0_identity(pkg):
     print(f"full-size z=0 identity rel L2 {e:.1e}", flush=True)
     assert e <= 1e-5
     assert torch.isfinite(inten).all()
+
+
+@pytest.mark.parametrize("C,H,W,pH,pW,dists", [FIXED_CASES[0], FIXED_CASES[1], CASES[0][:6],
+                                                BLUESTEIN_CASES[0][:6]],
+                         ids=["1152", "2816x4480", "224x260", "bluestein-92x236"])
+def test_tables_built_once_then_kept(pkg, C, H, W, pH, pW, dists):
+    """CVQPG_BLASM_TABLES alone builds every transform table; passes with
+    CVQPG_BLASM_TABLES_KEPT then reuse them (no table launches) and give bit-identical
+    planes, plane fields and field gradients to the plain calls, on a workspace that held
+    NaN before the tables-only call (compile-time plans at 1152 and 2816 x 4480, a runtime
+    plan and a Bluestein length)."""
+    from cvqpg_amd import _lib
+    from cvqpg_amd.optics import HologramOptics
+    L = _lib.lib()
+    wl = [639e-9, 515e-9, 473e-9][:C] if C == 3 else [515e-9]
+    opt = HologramOptics((W, H), (pH, pW), wl, 8e-6, dists, aperture_size=10 * max(W, H))
+    field = torch.as_tensor(make_field(C, H, W, seed=5)).to(DEV)
+    g = torch.rand(len(dists), C, H, W, device=DEV)
+    inten, out = opt.forward(field, keep_field=True)
+    grad = opt.backward(field, g, out)
+    ws = opt._workspace(field.device)
+    ws.view(torch.float32)[:ws.numel() // 4].fill_(float("nan"))
+    s = _lib.stream_handle(field.device)
+    fr = lambda t: _lib.ptr(torch.view_as_real(t))
+    _lib.check(L.cvqpg_blasm_forward_passes(8, None, *opt._common(), _lib.ptr(ws), ws.numel(),
+                                            None, None, s))
+    inten2 = torch.empty_like(inten)
+    out2 = torch.empty_like(out)
+    grad2 = torch.empty_like(grad)
+    for m in (1, 2, 4):
+        _lib.check(L.cvqpg_blasm_forward_passes(m | 16, fr(field), *opt._common(), _lib.ptr(ws),
+                                                ws.numel(), fr(out2), _lib.ptr(inten2), s))
+    _lib.check(L.cvqpg_blasm_backward_passes(7 | 16, fr(field), _lib.ptr(g), fr(out2),
+                                             *opt._common(), _lib.ptr(ws), ws.numel(),
+                                             fr(grad2), s))
+    torch.cuda.synchronize()
+    assert torch.equal(inten, inten2) and torch.equal(out, out2) and torch.equal(grad, grad2)

@@ -6,8 +6,8 @@ supervised steps, 400 render-loss fine-tuning steps on 8 further 256x256 scenes,
 validation choice between the fine-tuned and the supervised network, then two held-out
 scenes: the plain fit runs 400 iterations and its final per-plane PSNR is
 the bar; the warm-started fit (network Gaussians + random ones up to the same N) must
-reach every plane's bar minus 0.1 dB in at most half the time, and within one evaluation
-(25 iterations) of the image-independent start from the mean slot encoding.
+reach every plane's bar minus 0.1 dB in at most half the time, and in fewer iterations than
+the image-independent start from the mean slot encoding.
 """
 import pytest
 
@@ -30,6 +30,7 @@ def test_warm_start_reaches_plain_psnr_faster(pkg):
     print("net", e["net"], "mean", e["mean_start"], flush=True)
     assert e["net"]["warm_reached"] == 2
     assert e["net"]["median_time_ratio"] <= 0.5
-    # the image-blind mean-slot start is the network's own starting point (head bias);
-    # the trained network is not allowed to fall behind it by more than one evaluation
-    assert e["net"]["median_warm_iters"] <= e["mean_start"]["median_warm_iters"] + 25
+    # the image-blind mean-slot start is the network's own starting point (head bias): the
+    # trained network must reach the bar in fewer iterations than it (measured: 51 against
+    # 76, one evaluation ahead on both scenes)
+    assert e["net"]["median_warm_iters"] < e["mean_start"]["median_warm_iters"]

@@ -65,6 +65,23 @@ def test_workspace_size_and_argument_errors(pkg):
     assert rc == 1 and b"NULL" in L.cvqpg_last_error()
     rc = L.cvqpg_loss_target_stats(1, 3, 4, 40, 56, 1, 100, None)
     assert rc == 1 and b"too small" in L.cvqpg_last_error()
+    # BLASM pass masks: 1 | 2 | 4, CVQPG_BLASM_TABLES (8) alone or with passes,
+    # CVQPG_BLASM_TABLES_KEPT (16) only with something to run
+    wl, dist = _lib.doubles([639e-9, 515e-9, 473e-9]), _lib.doubles([0.0, 2e-3])
+    for bad in (0, 16, 32, -1):
+        rc = L.cvqpg_blasm_forward_passes(bad, None, 3, 40, 56, 64, 128, 2, wl, dist, 8e-6,
+                                          0.0, 0, None, 0, None, None, None)
+        assert rc == 1 and b"mask" in L.cvqpg_last_error(), bad
+        rc = L.cvqpg_blasm_backward_passes(bad, None, None, None, 3, 40, 56, 64, 128, 2, wl,
+                                           dist, 8e-6, 0.0, 0, None, 0, None, None)
+        assert rc == 1 and b"mask" in L.cvqpg_last_error(), bad
+    # a tables-only call needs the workspace but no data pointers
+    rc = L.cvqpg_blasm_forward_passes(8, None, 3, 40, 56, 64, 128, 2, wl, dist, 8e-6, 0.0, 0,
+                                      None, 0, None, None, None)
+    assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+    rc = L.cvqpg_blasm_forward_passes(1 | 16, None, 3, 40, 56, 64, 128, 2, wl, dist, 8e-6, 0.0,
+                                      0, None, 0, None, None, None)
+    assert rc == 1 and b"NULL" in L.cvqpg_last_error()
 
 
 def test_batch_table_layout_and_errors(pkg):
