@@ -24,6 +24,8 @@
 // full 2x spectrum never goes to HBM in either direction.  The transform lengths of the
 // standard workloads run on compile-time pipelines (k_*_fx, see fft::Pipe); any other
 // size runs on the runtime-plan kernels.
+#include <type_traits>
+
 #include "common.h"
 #include "cvqpg.h"
 #include "fft.h"
@@ -74,6 +76,7 @@ struct Optics {
   int or0, oc0, oh, ow;  // output crop on the 2x grid
   int gr0;               // first 2x-grid row of the hologram (adjoint output rows)
   int cg;                // columns per column workgroup
+  int zpair;             // adjoint row spectra Z stored as row pairs (see z_at)
   float norm;            // 1 / (nu nv)
   float fr0, frs;        // centred row-frequency grid  f = fr0 + i frs  (1/m)
   float fc0, fcs;        // centred column-frequency grid
@@ -175,6 +178,25 @@ __global__ __launch_bounds__(256) void k_bluestein_tables(int n, int m, float2* 
   }
 }
 
+// A table of N (compile-time) entries copied from global memory into LDS by NT threads.
+// load() issues every load of the thread before store() writes any of them, so the copy
+// costs one memory latency instead of one per round of NT entries, and a caller can issue
+// its own loads in between; the caller's next barrier publishes the table.
+template <int N, int NT>
+struct TableCopy {
+  static constexpr int R = N > 0 ? (N + NT - 1) / NT : 0;
+  float2 v[R > 0 ? R : 1];
+  __device__ __forceinline__ void load(const float2* __restrict__ src) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[min((int)threadIdx.x + r * NT, N - 1)];
+  }
+  __device__ __forceinline__ void store(float2* __restrict__ dst) const {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if ((int)threadIdx.x + r * NT < N) dst[threadIdx.x + r * NT] = v[r];
+  }
+};
+
 // copy the plan's twiddle table into LDS; the caller's next barrier publishes it
 __device__ __forceinline__ const float2* stage_twiddles(const fft::Plan& p, float2* stw) {
   const int ne = p.nh + fft::TW_LO;
@@ -232,6 +254,17 @@ __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int 
   }
   const float fr = __builtin_amdgcn_fractf(-o.zl[d][c] * g);
   return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
+}
+
+// Element (row r, column j) of plane-channel dc of the adjoint row spectra Z
+// [D C][oh][nv]: row-major, or with o.zpair as row pairs, rows 2p and 2p + 1 of a column
+// side by side, so that the prefetching column kernel (one column per workgroup) loads
+// both with one 16-byte access instead of two separate 8-byte pieces 8 nv bytes apart
+// (oh is even).
+__device__ __forceinline__ size_t z_at(const Optics& o, int dc, int r, int j) {
+  const size_t base = (size_t)dc * o.oh * o.nv;
+  return o.zpair ? base + ((size_t)(r >> 1) * o.nv + j) * 2 + (r & 1)
+                 : base + (size_t)r * o.nv + j;
 }
 
 // Row spectra (X, Z: hundreds of MB, read once by the next pass) are written with
@@ -353,8 +386,7 @@ __global__ __launch_bounds__(ROW_NT) void k_a3_rows(const float* __restrict__ gr
   }
   __syncthreads();
   F::template exec<1, 1, ROW_NT>(buf, plan, stw);
-  float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
-  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) dst[j] = buf[fft::lidx(j)];
+  for (int j = threadIdx.x; j < o.nv; j += ROW_NT) Z[z_at(o, dc, r, j)] = buf[fft::lidx(j)];
 }
 
 // A1: adjoint of F1 on the hologram rows: unnormalised inverse FFT along the row, the
@@ -672,29 +704,27 @@ __global__ __launch_bounds__(P::nt) void k_f3_rows_fx(const float2* __restrict__
   }
 }
 
-// A3 (see k_a3_rows) with a compile-time plan.  grid (oh, D*C)
-template <class P>
-__global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ grad_inten,
-                                                      const float2* __restrict__ out, Optics o,
-                                                      fft::Plan plan, float2* __restrict__ Z) {
-  constexpr int N = P::n, NT = P::nt;
+// A3 (see k_a3_rows) with a compile-time plan: the row spectra of rows r0 + g, g < CG, of
+// plane-channel dc (CG interleaved sequences in buf) into the last-stage registers v
+// (synchronised on entry; ends with buf still being read)
+template <class P, int CG, int NT>
+__device__ __forceinline__ void a3_rows_fx(
+    const float* __restrict__ grad_inten, const float2* __restrict__ out, const Optics& o,
+    const fft::Plan& plan, const float2* stw, float2* buf, int r0, int dc,
+    float2 (&v)[P::Fwd::template GL<CG, NT>::MAXB][P::Fwd::LAST]) {
   using Fw = typename P::Fwd;
-  using G1 = typename Fw::template G1<1, NT>;
-  using GL = typename Fw::template GL<1, NT>;
+  using G1 = typename Fw::template G1<CG, NT>;
   static_assert(Fw::FIRST % 4 == 0, "first radix must be divisible by 4");
-  extern __shared__ float2 buf[];
-  const float2* stw = stage_twiddles(plan, buf + P::lds);
-  const int r = blockIdx.x, dc = blockIdx.y;
-  const size_t ob = ((size_t)dc * o.oh + r) * o.ow;
   const float2 cr = o.carrier[dc / o.C][dc % o.C];
   const float2 crc = make_float2(cr.x, -cr.y);
   {
-    float2 v[G1::MAXB][Fw::FIRST];
+    float2 u[G1::MAXB][Fw::FIRST];
 #pragma unroll
     for (int b = 0; b < G1::MAXB; ++b) {
       if (!G1::live(b)) continue;
-      int j, col;
-      G1::item(b, j, col);
+      int j, g;
+      G1::item(b, j, g);
+      const size_t ob = ((size_t)dc * o.oh + r0 + g) * o.ow;
       // every load of the row is issued before any is used (a load guarded together with
       // its use would wait for each in turn); out-of-crop elements read element 0
       float gi[Fw::FIRST];
@@ -709,32 +739,108 @@ __global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ 
       }
 #pragma unroll
       for (int k = 0; k < Fw::FIRST; ++k) {
-        float2 g = make_float2(0.f, 0.f);
+        float2 q = make_float2(0.f, 0.f);
         if (in_mid_half<Fw>(k)) {
           const int x = j + k * G1::CNT - o.oc0;
           if ((unsigned)x < (unsigned)o.ow) {
             const float sc = 2.0f * gi[k] * o.norm;
             const float2 f = fft::cmul(fo[k], crc);
-            g = make_float2(sc * f.x, sc * f.y);
+            q = make_float2(sc * f.x, sc * f.y);
           }
         }
-        v[b][k] = g;
+        u[b][k] = q;
       }
     }
-    Fw::template first<1, 1, NT>(buf, v);
+    Fw::template first<1, CG, NT>(buf, u);
   }
   __syncthreads();
-  Fw::template middle<1, 1, NT>(buf, stw, plan.nh);
+  Fw::template middle<1, CG, NT>(buf, stw, plan.nh);
+  Fw::template last<1, CG, NT>(buf, stw, plan.nh, v);
+}
+
+// A3 with a compile-time plan, row-major Z: one row per workgroup.  grid (oh, D*C)
+template <class P>
+__global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ grad_inten,
+                                                      const float2* __restrict__ out, Optics o,
+                                                      fft::Plan plan, float2* __restrict__ Z) {
+  constexpr int NT = P::nt;
+  using Fw = typename P::Fwd;
+  using GL = typename Fw::template GL<1, NT>;
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + P::lds);
+  const int r = blockIdx.x, dc = blockIdx.y;
   float2 v[GL::MAXB][Fw::LAST];
-  Fw::template last<1, 1, NT>(buf, stw, plan.nh, v);
-  float2* dst = Z + ((size_t)dc * o.oh + r) * o.nv;
+  a3_rows_fx<P, 1, NT>(grad_inten, out, o, plan, stw, buf, r, dc, v);
+  float2* dst = Z + z_at(o, dc, r, 0);
 #pragma unroll
   for (int b = 0; b < GL::MAXB; ++b) {
     if (!GL::live(b)) continue;
-    int j, col;
-    GL::item(b, j, col);
+    int j, g;
+    GL::item(b, j, g);
 #pragma unroll
     for (int k = 0; k < Fw::LAST; ++k) stream_store(dst + j + k * GL::CNT, v[b][k]);
+  }
+}
+
+// A3 with a compile-time plan and Z as row pairs (o.zpair): rows 2p and 2p + 1 per
+// workgroup, so every line of Z is written whole.  When twice the plan's threads fit a
+// workgroup, the two rows are transformed together as two interleaved sequences
+// (2 nt threads; lanes 2m and 2m + 1 hold the two rows of one column, which are adjacent
+// in Z, so a wave's stores are contiguous); otherwise one after the other on nt threads,
+// the first row's spectrum kept in registers and each column pair stored as one 16-byte
+// piece.  grid (oh / 2, D*C)
+template <class P>
+constexpr bool a3_pair_together() {
+  return 2 * P::nt <= 1024 &&
+         (P::Fwd::lds_elems(2) + fft::twiddle_entries(P::n)) * sizeof(float2) <= 80 * 1024;
+}
+template <class P>
+constexpr int a3_pair_threads() { return a3_pair_together<P>() ? 2 * P::nt : P::nt; }
+template <class P>
+constexpr int a3_pair_lds() { return a3_pair_together<P>() ? P::Fwd::lds_elems(2) : P::lds; }
+
+template <class P>
+__global__ __launch_bounds__(a3_pair_threads<P>()) void k_a3_rows_pair_fx(
+    const float* __restrict__ grad_inten, const float2* __restrict__ out, Optics o,
+    fft::Plan plan, float2* __restrict__ Z) {
+  using Fw = typename P::Fwd;
+  extern __shared__ float2 buf[];
+  const float2* stw = stage_twiddles(plan, buf + a3_pair_lds<P>());
+  const int r = 2 * blockIdx.x, dc = blockIdx.y;
+  if constexpr (a3_pair_together<P>()) {
+    constexpr int NT = 2 * P::nt;
+    using GL = typename Fw::template GL<2, NT>;
+    float2 v[GL::MAXB][Fw::LAST];
+    a3_rows_fx<P, 2, NT>(grad_inten, out, o, plan, stw, buf, r, dc, v);
+#pragma unroll
+    for (int b = 0; b < GL::MAXB; ++b) {
+      if (!GL::live(b)) continue;
+      int j, g;
+      GL::item(b, j, g);
+      float2* dst = Z + z_at(o, dc, r, j) + g;   // row r + g, column j
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k) stream_store(dst + 2 * k * GL::CNT, v[b][k]);
+    }
+  } else {
+    constexpr int NT = P::nt;
+    using GL = typename Fw::template GL<1, NT>;
+    float2 v[GL::MAXB][Fw::LAST], w[GL::MAXB][Fw::LAST];
+    a3_rows_fx<P, 1, NT>(grad_inten, out, o, plan, stw, buf, r, dc, v);
+    __syncthreads();   // every read of buf by the first row's last stage is complete
+    a3_rows_fx<P, 1, NT>(grad_inten, out, o, plan, stw, buf, r + 1, dc, w);
+    typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int b = 0; b < GL::MAXB; ++b) {
+      if (!GL::live(b)) continue;
+      int j, g;
+      GL::item(b, j, g);
+      f4* dst = reinterpret_cast<f4*>(Z + z_at(o, dc, r, j));
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k) {
+        const f4 q = {v[b][k].x, v[b][k].y, w[b][k].x, w[b][k].y};
+        __builtin_nontemporal_store(q, dst + k * GL::CNT);
+      }
+    }
   }
 }
 
@@ -829,7 +935,8 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
   const int c = blockIdx.y;
   const float2* stw = stage_twiddles(plan, buf + P::lds);
   float2* tt = buf + col_lds_head<P>();
-  for (int i = threadIdx.x; i < Iv::TABLE; i += NT) tt[i] = tabs[Fw::TABLE + i];
+  TableCopy<Iv::TABLE, NT> tcopy;
+  tcopy.load(tabs + Fw::TABLE);   // stored after the column's loads are issued
   // plane bases are uniform; offsets inside a plane (< 2^32 elements) are 32-bit
   const float2* Xc = X + (size_t)c * o.pH * o.nv + col0;
   {
@@ -849,6 +956,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
         v[b][k] = u;
       }
     }
+    tcopy.store(tt);
     Fw::template first<1, CG, NT>(buf, v);
   }
   __syncthreads();
@@ -928,7 +1036,11 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
   const int c = blockIdx.y;
   const float2* stw = stage_twiddles(plan, buf + P::lds);
   float2* tt = buf + col_lds_head<P>();
-  for (int i = threadIdx.x; i < Fw::TABLE; i += NT) tt[i] = tabs[i];
+  {
+    TableCopy<Fw::TABLE, NT> tcopy;
+    tcopy.load(tabs);
+    tcopy.store(tt);
+  }
   float2 A[FL::MAXB][Fw::LAST];
 #if !CVQPG_PG_RECOMPUTE_A
   float Pg[FL::MAXB][Fw::LAST];
@@ -1039,26 +1151,30 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_pf(const float2* __
   __shared__ float2 buf[P::lds];
   __shared__ float2 stws[NTW];
   __shared__ float2 tt[TM];
-  __shared__ float pfs[2 * HALF];              // real parts [HALF], imaginary parts [HALF]
+  __shared__ float2 pfs[HALF];                 // the next plane's column, rows [0, oh)
   const int col0 = xcd_block(blockIdx.x, per);
   if (col0 >= o.nv) return;
   const int c = blockIdx.y;
   const float2* stw = stage_twiddles(plan, stws);
-  for (int i = threadIdx.x; i < TM; i += NT) tt[i] = tabs[i];
-  // rows i < oh of plane d's column: wave w of round i0 covers rows i0 + 64 w + lane
+  {
+    TableCopy<TM, NT> tcopy;
+    tcopy.load(tabs);
+    tcopy.store(tt);
+  }
+  // rows [0, oh) of plane d's column, stored as row pairs (z_at): lane l of wave w in
+  // round p0 loads pair p = p0 + 64 w + l, rows 2p and 2p + 1, as one 16-byte piece into
+  // pfs[2p], pfs[2p + 1] (global_load_lds_dwordx4: lane l writes bytes [16 l, 16 l + 16)
+  // after the wave's LDS base)
   auto prefetch = [&](int d) {
-    const float* src =
-        reinterpret_cast<const float*>(Z + ((size_t)d * o.C + c) * o.oh * o.nv + col0);
-    for (int i0 = 0; i0 < o.oh; i0 += NT) {
-      const int i = i0 + (int)threadIdx.x;
-      const int wb = __builtin_amdgcn_readfirstlane(i0 + ((int)threadIdx.x & ~63));
-      if (i < o.oh) {
-        const float* g = src + 2 * (size_t)((unsigned)(i * o.nv));
-        __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(pfs + wb),
-                                         4, 0, 0);
-        __builtin_amdgcn_global_load_lds(
-            g + 1, (__attribute__((address_space(3))) void*)(pfs + HALF + wb), 4, 0, 0);
-      }
+    const float2* src = Z + z_at(o, d * o.C + c, 0, col0);
+    const int np = o.oh >> 1;
+    for (int p0 = 0; p0 < np; p0 += NT) {
+      const int p = p0 + (int)threadIdx.x;
+      const int wb = __builtin_amdgcn_readfirstlane(p0 + ((int)threadIdx.x & ~63));
+      if (p < np)
+        __builtin_amdgcn_global_load_lds(src + 2 * (size_t)((unsigned)(p * o.nv)),
+                                         (__attribute__((address_space(3))) void*)(pfs + 2 * wb),
+                                         16, 0, 0);
     }
   };
   float2 A[FL::MAXB][Fw::LAST];
@@ -1088,7 +1204,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_pf(const float2* __
           float2 u = make_float2(0.f, 0.f);
           if (in_mid_half<Fw>(k)) {
             const int rr = j + k * F1::CNT - o.or0;
-            if ((unsigned)rr < (unsigned)o.oh) u = make_float2(pfs[rr], pfs[HALF + rr]);
+            if ((unsigned)rr < (unsigned)o.oh) u = pfs[rr];
           }
           v[b][k] = u;
         }
@@ -1227,10 +1343,35 @@ static void launch_fx(Kern k, dim3 grid, size_t sh, hipStream_t s, Args... args)
   hipLaunchKernelGGL(k, grid, dim3(P::nt), sh, s, args...);
 }
 
+template <class P>
+static void launch_a3_fx(const Optics& o, const BlasmWS& w, const fft::Plan& pc,
+                         const float* grad_intensity, const float2* of, hipStream_t s) {
+  if (o.zpair) {
+    constexpr size_t sh =
+        (size_t)(a3_pair_lds<P>() + fft::twiddle_entries(P::n)) * sizeof(float2);
+    allow_lds(k_a3_rows_pair_fx<P>, sh);
+    hipLaunchKernelGGL(k_a3_rows_pair_fx<P>, dim3(o.oh / 2, o.D * o.C),
+                       dim3(a3_pair_threads<P>()), sh, s, grad_intensity, of, o, pc, w.Y);
+  } else {
+    const size_t sh = (size_t)(P::lds + fft::twiddle_entries(P::n)) * sizeof(float2);
+    allow_lds(k_a3_rows_fx<P>, sh);
+    hipLaunchKernelGGL(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), dim3(P::nt), sh, s,
+                       grad_intensity, of, o, pc, w.Y);
+  }
+}
+
 static int pick_cg(int nu, int nv) {
   for (int cg = 8; cg >= 1; cg >>= 1)
     if ((int64_t)nu * cg <= COL_LCAP && nv % cg == 0) return cg;
   return 0;
+}
+
+// Whether the adjoint column pass of a geometry runs the prefetching kernel
+// (k_a2_cols_pf), i.e. launch_cols picks ColP2816A (column group 2, length 2816) or
+// ColP4992 (column group 1, length 4992), both of one column per workgroup; that kernel
+// reads the row spectra Z as row pairs, which the adjoint row pass then writes.
+static bool adj_cols_prefetch(int nu, int cg) {
+  return (cg == 2 && nu == 2816) || (cg == 1 && nu == 4992);
 }
 
 static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
@@ -1262,6 +1403,7 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   o.cg = pick_cg(mu, o.nv);
   CVQPG_CHECK_ARG(o.cg > 0, "padded height %d exceeds the supported column length "
                   "(transform length %d, Stockham length %d > %d)", pH, o.nu, mu, COL_LCAP);
+  o.zpair = adj_cols_prefetch(o.nu, o.cg) ? 1 : 0;
   // the pruned transforms rely on every non-zero input and every kept output lying in
   // the centre half [n/4, 3n/4) of the 2x grid
   CVQPG_CHECK_ARG(o.zr0 == o.nu / 4 && o.zc0 == o.nv / 4 && o.or0 >= o.nu / 4 &&
@@ -1366,6 +1508,8 @@ static void launch_cols_fx(const BlasmWS& w, const Optics& o, const fft::Plan& p
   const dim3 grid(8 * per, o.C);
   const float2* tabs = w.tw_cs;
   if constexpr (ADJ && P::cg == 1) {   // all LDS of the prefetching kernel is static
+    static_assert(std::is_same<P, ColP2816A>::value || std::is_same<P, ColP4992>::value,
+                  "adj_cols_prefetch must name every plan of the prefetching kernel");
     if (o.band_active)
       launch_fx<P>(k_a2_cols_pf<P, true>, grid, 0, s, (const float2*)w.Y, o, pr, tabs, per, w.X);
     else
@@ -1577,8 +1721,7 @@ extern "C" int cvqpg_blasm_backward_passes(int passes, const float* field,
     const float2* of = reinterpret_cast<const float2*>(out_field);
     const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
       using P = decltype(pl);
-      launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s,
-                   grad_intensity, of, o, pc, w.Y);
+      launch_a3_fx<P>(o, w, pc, grad_intensity, of, s);
     });
     if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, grad_intensity, of, o, pc, w.Y);
     CVQPG_LAUNCH_CHECK();
@@ -1657,8 +1800,7 @@ extern "C" int cvqpg_blasm_reconstruct_backward(
   const float2* of = reinterpret_cast<const float2*>(out_field);
   const bool fx3 = with_row_plan(o.nv, [&](auto pl) {
     using P = decltype(pl);
-    launch_fx<P>(k_a3_rows_fx<P>, dim3(o.oh, o.D * o.C), row_fx_lds_bytes<P>(), s, grad_intensity,
-                 of, o, pc, w.Y);
+    launch_a3_fx<P>(o, w, pc, grad_intensity, of, s);
   });
   if (!fx3) ROW_LAUNCH(k_a3_rows, pc, dim3(o.oh, o.D * o.C), row_lds_bytes(pc), s, grad_intensity, of, o, pc, w.Y);
   launch_cols_for(w, o, pr, s, true, COLS_RUN);

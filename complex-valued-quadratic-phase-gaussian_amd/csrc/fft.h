@@ -48,7 +48,7 @@ struct Plan {
 };
 
 constexpr int TW_LO = 64;
-__host__ __device__ inline int twiddle_entries(int n) { return (n + TW_LO - 1) / TW_LO + TW_LO; }
+__host__ __device__ constexpr int twiddle_entries(int n) { return (n + TW_LO - 1) / TW_LO + TW_LO; }
 
 // LDS element index with one padding slot every 16 complex elements, so the strided
 // accesses of the early Stockham stages spread over the banks.
