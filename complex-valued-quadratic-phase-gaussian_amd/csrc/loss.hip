@@ -147,6 +147,10 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   __shared__ float smo[2][NM][NIN];  // staged input row
   __shared__ float smap[2][3][NTH];
   __shared__ float red[2][NTH];
+  // raw (intensity, target) of the thread's input column mc for the last 11 input rows,
+  // by step modulo 11: pixel row y - 10 of a step was staged as input row 10 steps
+  // earlier by the same thread, so its values are not loaded from HBM a second time
+  __shared__ float2 pring[WIN][NTH];
   const int i = threadIdx.x;
   const int x0 = blockIdx.x * P.own;
   const int p0 = blockIdx.y * P.rs;
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   const int gx0 = mc, gx1 = mc + NTH;
   const bool two = i < WIN - 1;
   float pa0 = 0.f, pb0 = 0.f, pa1 = 0.f, pb1 = 0.f;   // next input row (raw inten, target)
-  float qI = 0.f, qT = 0.f, qM = 0.f, qN = 0.f;       // next pixel row values
+  float qM = 0.f, qN = 0.f;                           // next pixel row values
   auto load_row = [&](int y) {
     pa0 = pb0 = pa1 = pb1 = 0.f;
     if (y >= 0 && y < P.H) {
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   auto load_pix = [&](int pr) {
     if (pix_col && pr >= p0 && pr < p_end) {
       const int r = pr * Wd + mc;
-      qI = inp[r]; qT = tgp[r]; qM = mkp[r];
+      qM = mkp[r];
       if (nzp) qN = nzp[r];
     }
   };
@@ -222,6 +226,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       {
         const float a = clamp01(pa0), b = pb0;
         smo[buf][0][i] = a; smo[buf][1][i] = a * a; smo[buf][2][i] = a * b;
+        pring[k][i] = make_float2(pa0, pb0);
         if (!TS) { smo[buf][NM - 2][i] = b; smo[buf][NM - 1][i] = b * b; }
         if (two) {
           const float a1 = clamp01(pa1), b1 = pb1;
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
           if (!TS) { smo[buf][NM - 2][i + NTH] = b1; smo[buf][NM - 1][i + NTH] = b1 * b1; }
         }
       }
-      const float cI = qI, cT = qT, cM = qM, cN = qN;
+      const float cM = qM, cN = qN;
       const float2 cS = sN;
       load_row(y + 1);
       load_pix(y - (WIN - 1) + 1);
@@ -312,6 +317,8 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
         normals4(P.seed, step, (uint32_t)d,
                  ((uint32_t)c * hq + (uint32_t)(pr >> 2)) * (uint32_t)Wd + (uint32_t)mc, z4);
       if (!pix_col) continue;
+      const float2 pit = pring[(k + 1) % WIN][i];   // staged at step sidx - 10: row pr
+      const float cI = pit.x, cT = pit.y;
       float gA = 0.f, gB = 0.f, gC = 0.f;
 #pragma unroll
       for (int a = 0; a < WIN; ++a) {

@@ -226,3 +226,27 @@ def test_tables_built_once_then_kept(pkg, C, H, W, pH, pW, dists):
                                              fr(grad2), s))
     torch.cuda.synchronize()
     assert torch.equal(inten, inten2) and torch.equal(out, out2) and torch.equal(grad, grad2)
+
+
+def test_row_pair_layout_with_runtime_row_plan(pkg):
+    """A 2816-row grid whose 320-point rows run on the runtime-plan row kernels: the adjoint
+    row spectra are stored as row pairs for the prefetching column kernel, written here by
+    the runtime adjoint row pass.  Forward and adjoint vs FP64, and the full-plane
+    reconstruct backward (oh = 1408, the whole centre half) vs FP64 autograd."""
+    test_forward_and_adjoint_match_fp64(pkg, 1, 1080, 96, 1408, 160, [0.0, 3e-3], "odak")
+    from cvqpg_amd.optics import HologramOptics, reconstruct_planes
+    C, pH, pW, zs = 1, 1408, 160, [2e-3]
+    rng = np.random.default_rng(9)
+    ph = torch.as_tensor(rng.uniform(0, 2 * np.pi, (C, pH, pW)), dtype=torch.float32)
+    am = torch.as_tensor(rng.uniform(0.2, 1.0, (C, pH, pW)), dtype=torch.float32)
+    wgt = torch.as_tensor(rng.normal(size=(len(zs), C, pH, pW)), dtype=torch.float32)
+    opt = HologramOptics((pW, pH), (pH, pW), [515e-9], 8e-6, zs, aperture_size=1e6)
+    php, amp = ph.to(DEV).requires_grad_(), am.to(DEV).requires_grad_()
+    (reconstruct_planes(php, amp, opt) * wgt.to(DEV)).sum().backward()
+    prop = ooptics.BLASM((pH, pW), [515e-9], 8e-6, zs, 1e6, "odak")
+    ph64, am64 = ph.double().requires_grad_(), am.double().requires_grad_()
+    (prop.reconstruct(torch.polar(am64, ph64)) * wgt.double()).sum().backward()
+    eg_p, eg_a = rel(php.grad, ph64.grad), rel(amp.grad, am64.grad)
+    print(f"2816 x 320 full-plane reconstruct gradients vs FP64: phase {eg_p:.1e}, "
+          f"amplitude {eg_a:.1e}", flush=True)
+    assert eg_p <= 2e-5 and eg_a <= 2e-5
