@@ -117,6 +117,23 @@ def gauss_window(device, dtype=torch.float32):
     return g / g.sum()
 
 
+def _conv_same(img, ker):
+    """2-D correlation of img [H, W] with the odd L x L kernel ker under zero padding (what
+    F.conv2d(img[None, None], ker[None, None], padding="same") computes), as a sum of
+    L^2 shifted copies: the library convolution picks a naive direct kernel for this
+    shape on some devices (hundreds of ms per 1080p plane)."""
+    L = ker.shape[0]
+    r = L // 2
+    H, W = img.shape
+    pad = F.pad(img[None, None], (r, r, r, r))[0, 0]
+    kv = ker.detach().cpu().tolist()
+    out = torch.zeros_like(img)
+    for a in range(L):
+        for b in range(L):
+            out.add_(pad[a:a + H, b:b + W], alpha=kv[a][b])
+    return out
+
+
 def _gfilter(x, w):
     C, n = x.shape[1], w.numel()
     x = F.conv2d(x, w.view(1, 1, 1, n).expand(C, 1, 1, n), groups=C)
@@ -188,21 +205,24 @@ class multiplane_loss_odak:  # noqa: N801 (reference class name)
                 self.masks[i, ch] = m
 
     def add_defocus_blur(self):
-        """utils/propagator.py:392-415."""
+        """utils/propagator.py:392-415.  The blurred image depends only on the channel and
+        the blur level s = int(|i - j| blur_ratio), so each (channel, s) is filtered once
+        and reused for every plane pair with that level."""
         L = self.target_blur_size
         P = self.number_of_planes
         for ch in range(self.target_image.shape[0]):
             cache = self.targets[:, ch].clone()
             total = cache.sum(0)
+            blurred = {}
             for i in range(P):
                 acc = torch.zeros_like(cache[i])
                 for j in range(P):
                     if torch.sum(cache[j]) > 0:
                         s = 0.0 if i == j else float(int(abs(i - j) * self.blur_ratio))
-                        ker = odak_gaussian_2d(L, s, self.device)
-                        ker = (ker / ker.sum())[None, None]
-                        bl = F.conv2d(total[None, None], ker, padding="same")[0, 0]
-                        acc = acc + bl * self.masks[j, ch].abs()
+                        if s not in blurred:
+                            ker = odak_gaussian_2d(L, s, self.device)
+                            blurred[s] = _conv_same(total, ker / ker.sum())
+                        acc = acc + blurred[s] * self.masks[j, ch].abs()
                 self.targets[i, ch] = acc
         self.targets = self.targets.detach().clone() * self.multiplier
 
