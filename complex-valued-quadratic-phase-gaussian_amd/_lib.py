@@ -61,6 +61,12 @@ SIGNATURES = {
     "cvqpg_blasm_reconstruct_backward": (_c_int, [_p] * 4 + [_c_int] * 4 + [_p, _p, _c_f64, _c_f64,
                                                                             _c_int, _p, _c_size,
                                                                             _p, _p, _p]),
+    "cvqpg_blasm_reconstruct_aperture": (_c_int, [_p, _p, _p] + [_c_int] * 4 + [_p, _p, _c_f64,
+                                                                             _c_int, _p, _c_size,
+                                                                             _p, _p, _p]),
+    "cvqpg_blasm_reconstruct_aperture_backward": (_c_int, [_p] * 5 + [_c_int] * 4 +
+                                                  [_p, _p, _c_f64, _c_int, _p, _c_size, _p, _p,
+                                                   _p]),
     "cvqpg_loss_workspace_size": (_c_size, [_c_int] * 4),
     "cvqpg_loss_forward_backward": (_c_int, [_p, _p, _p, _p, _p, ctypes.c_uint64, _p, _c_int,
                                              _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),

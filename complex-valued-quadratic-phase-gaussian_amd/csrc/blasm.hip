@@ -88,6 +88,7 @@ struct Optics {
   int ap_open;
   int band_active;       // 0 when no band limit and no evanescence can occur on the grid
   float ap_r2, apr0, aprs, apc0, apcs;
+  const float* ap_tab;   // custom aperture [nu][nv] on the centred grid, or NULL
 };
 
 struct BlasmWS {
@@ -254,6 +255,18 @@ __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int 
   }
   const float fr = __builtin_amdgcn_fractf(-o.zl[d][c] * g);
   return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
+}
+
+// Factor a^2 of a custom aperture a at the spectral point (kr, kc): the reference's
+// custom() multiplies the shifted spectrum and the transfer function by the aperture
+// (utils/propagator.py:59-61), and a is real, so the forward spectrum and, in the
+// adjoint, the plane sum are multiplied by a^2 once for every plane.
+__device__ __forceinline__ float ap_square(const Optics& o, int kr, int kc) {
+  int ir, ic;
+  float fy, fx;
+  tf_freqs(o, kr, kc, ir, ic, fy, fx);
+  const float a = o.ap_tab[(size_t)ir * o.nv + ic];
+  return a * a;
 }
 
 // Element (row r, column j) of plane-channel dc of the adjoint row spectra Z
@@ -473,6 +486,7 @@ __global__ __launch_bounds__(COL_NT) void k_f2_cols(const float2* __restrict__ X
     if (e < total) {
       S[b] = buf[fft::lidx(e)];
       P[b] = tf_point(o, c, e / CG, col0 + e % CG);
+      if (o.ap_tab) S[b] = fft::cscale(S[b], ap_square(o, e / CG, col0 + e % CG));
     }
   }
   for (int d = 0; d < o.D; ++d) {
@@ -537,7 +551,8 @@ __global__ __launch_bounds__(COL_NT) void k_a2_cols(const float2* __restrict__ Z
 #pragma unroll
   for (int b = 0; b < COL_PER; ++b) {
     const int e = threadIdx.x + b * COL_NT;
-    if (e < total) buf[fft::lidx(e)] = A[b];
+    if (e < total)
+      buf[fft::lidx(e)] = o.ap_tab ? fft::cscale(A[b], ap_square(o, e / CG, col0 + e % CG)) : A[b];
   }
   __syncthreads();
   F::template exec<-1, CG, COL_NT>(buf, plan, stw);
@@ -963,6 +978,17 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
   Fw::template middle<1, CG, NT>(buf, stw, plan.nh);
   float2 S[FL::MAXB][Fw::LAST];
   Fw::template last<1, CG, NT>(buf, stw, plan.nh, S);
+  if (o.ap_tab) {
+#pragma unroll
+    for (int b = 0; b < FL::MAXB; ++b) {
+      if (!FL::live(b)) continue;
+      int j, col;
+      FL::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k)
+        S[b][k] = fft::cscale(S[b][k], ap_square(o, j + k * FL::CNT, col0 + col));
+    }
+  }
 #if !CVQPG_PG_RECOMPUTE
   float Pg[FL::MAXB][Fw::LAST];
 #pragma unroll
@@ -1102,6 +1128,17 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_fx(const float2* __
     }
   }
   __syncthreads();
+  if (o.ap_tab) {
+#pragma unroll
+    for (int b = 0; b < FL::MAXB; ++b) {
+      if (!FL::live(b)) continue;
+      int j, col;
+      FL::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k)
+        A[b][k] = fft::cscale(A[b][k], ap_square(o, j + k * FL::CNT, col0 + col));
+    }
+  }
   Iv::template first<-1, CG, NT>(buf, A);
   __syncthreads();
   Iv::template middle<-1, CG, NT>(buf, stw, plan.nh);
@@ -1230,6 +1267,17 @@ __global__ __launch_bounds__(P::nt, P::wpe_a) void k_a2_cols_pf(const float2* __
     }
   }
   fft::lds_barrier();
+  if (o.ap_tab) {
+#pragma unroll
+    for (int b = 0; b < FL::MAXB; ++b) {
+      if (!FL::live(b)) continue;
+      int j, col;
+      FL::item(b, j, col);
+#pragma unroll
+      for (int k = 0; k < Fw::LAST; ++k)
+        A[b][k] = fft::cscale(A[b][k], ap_square(o, j + k * FL::CNT, col0 + col));
+    }
+  }
   Iv::template first<-1, CG, NT>(buf, A);
   fft::lds_barrier();
   Iv::template middle<-1, CG, NT>(buf, stw, plan.nh);
@@ -1460,6 +1508,7 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
   // circular aperture on the centred grid: linspace(-n/2, n/2, n) per axis
   const double rmax = sqrt((o.nu / 2.0) * (o.nu / 2.0) + (o.nv / 2.0) * (o.nv / 2.0));
   o.ap_open = !(aperture_radius > 0) || aperture_radius > rmax + 1.0;
+  o.ap_tab = nullptr;
   o.ap_r2 = (float)(aperture_radius * aperture_radius);
   o.apr0 = (float)(-o.nu / 2.0);
   o.aprs = (float)((double)o.nu / (o.nu - 1));
@@ -1756,16 +1805,16 @@ extern "C" int cvqpg_blasm_backward(const float* field, const float* grad_intens
                                      stream);
 }
 
-extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitude, int channels,
-                                       int pad_height, int pad_width, int planes,
-                                       const double* wavelengths, const double* distances,
-                                       double pixel_pitch, double aperture_radius, int grid,
-                                       void* ws, size_t ws_bytes, float* out_field,
-                                       float* intensity, void* stream) {
+static int reconstruct_impl(const float* phase, const float* amplitude, const float* aperture,
+                            int channels, int pad_height, int pad_width, int planes,
+                            const double* wavelengths, const double* distances,
+                            double pixel_pitch, double aperture_radius, int grid, void* ws,
+                            size_t ws_bytes, float* out_field, float* intensity, void* stream) {
   Optics o;
   if (int rc = make_optics(o, channels, pad_height, pad_width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 1))
     return rc;
+  o.ap_tab = aperture;
   CVQPG_CHECK_ARG(phase && amplitude && intensity && ws, "NULL argument pointer");
   const BlasmWS need = carve_blasm(nullptr, o.C, o.D, o.H, o.pH, o.pW, o.oh);
   CVQPG_CHECK_ARG(ws_bytes >= need.bytes, "workspace too small: %zu < %zu", ws_bytes, need.bytes);
@@ -1778,16 +1827,17 @@ extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitud
   return forward_cols_rows(w, o, pr, pc, out_field, intensity, s);
 }
 
-extern "C" int cvqpg_blasm_reconstruct_backward(
-    const float* phase, const float* amplitude, const float* grad_intensity,
-    const float* out_field, int channels, int pad_height, int pad_width, int planes,
-    const double* wavelengths, const double* distances, double pixel_pitch,
-    double aperture_radius, int grid, void* ws, size_t ws_bytes, float* grad_phase,
-    float* grad_amplitude, void* stream) {
+static int reconstruct_backward_impl(
+    const float* phase, const float* amplitude, const float* aperture,
+    const float* grad_intensity, const float* out_field, int channels, int pad_height,
+    int pad_width, int planes, const double* wavelengths, const double* distances,
+    double pixel_pitch, double aperture_radius, int grid, void* ws, size_t ws_bytes,
+    float* grad_phase, float* grad_amplitude, void* stream) {
   Optics o;
   if (int rc = make_optics(o, channels, pad_height, pad_width, pad_height, pad_width, planes,
                            wavelengths, distances, pixel_pitch, aperture_radius, grid, 1))
     return rc;
+  o.ap_tab = aperture;
   CVQPG_CHECK_ARG(phase && amplitude && grad_intensity && out_field && grad_phase &&
                       grad_amplitude && ws,
                   "NULL argument pointer");
@@ -1808,4 +1858,51 @@ extern "C" int cvqpg_blasm_reconstruct_backward(
              amplitude, o, pc, grad_phase, grad_amplitude);
   CVQPG_LAUNCH_CHECK();
   return 0;
+}
+
+extern "C" int cvqpg_blasm_reconstruct(const float* phase, const float* amplitude, int channels,
+                                       int pad_height, int pad_width, int planes,
+                                       const double* wavelengths, const double* distances,
+                                       double pixel_pitch, double aperture_radius, int grid,
+                                       void* ws, size_t ws_bytes, float* out_field,
+                                       float* intensity, void* stream) {
+  return reconstruct_impl(phase, amplitude, nullptr, channels, pad_height, pad_width, planes,
+                          wavelengths, distances, pixel_pitch, aperture_radius, grid, ws,
+                          ws_bytes, out_field, intensity, stream);
+}
+
+extern "C" int cvqpg_blasm_reconstruct_backward(
+    const float* phase, const float* amplitude, const float* grad_intensity,
+    const float* out_field, int channels, int pad_height, int pad_width, int planes,
+    const double* wavelengths, const double* distances, double pixel_pitch,
+    double aperture_radius, int grid, void* ws, size_t ws_bytes, float* grad_phase,
+    float* grad_amplitude, void* stream) {
+  return reconstruct_backward_impl(phase, amplitude, nullptr, grad_intensity, out_field,
+                                   channels, pad_height, pad_width, planes, wavelengths,
+                                   distances, pixel_pitch, aperture_radius, grid, ws, ws_bytes,
+                                   grad_phase, grad_amplitude, stream);
+}
+
+extern "C" int cvqpg_blasm_reconstruct_aperture(
+    const float* phase, const float* amplitude, const float* aperture, int channels,
+    int pad_height, int pad_width, int planes, const double* wavelengths,
+    const double* distances, double pixel_pitch, int grid, void* ws, size_t ws_bytes,
+    float* out_field, float* intensity, void* stream) {
+  CVQPG_CHECK_ARG(aperture, "NULL aperture");
+  return reconstruct_impl(phase, amplitude, aperture, channels, pad_height, pad_width, planes,
+                          wavelengths, distances, pixel_pitch, 0.0, grid, ws, ws_bytes,
+                          out_field, intensity, stream);
+}
+
+extern "C" int cvqpg_blasm_reconstruct_aperture_backward(
+    const float* phase, const float* amplitude, const float* aperture,
+    const float* grad_intensity, const float* out_field, int channels, int pad_height,
+    int pad_width, int planes, const double* wavelengths, const double* distances,
+    double pixel_pitch, int grid, void* ws, size_t ws_bytes, float* grad_phase,
+    float* grad_amplitude, void* stream) {
+  CVQPG_CHECK_ARG(aperture, "NULL aperture");
+  return reconstruct_backward_impl(phase, amplitude, aperture, grad_intensity, out_field,
+                                   channels, pad_height, pad_width, planes, wavelengths,
+                                   distances, pixel_pitch, 0.0, grid, ws, ws_bytes, grad_phase,
+                                   grad_amplitude, stream);
 }

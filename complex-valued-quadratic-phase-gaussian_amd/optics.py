@@ -15,7 +15,10 @@ from . import _lib
 
 class HologramOptics:
     def __init__(self, img_size, pad_size, wavelengths, pixel_pitch, distances,
-                 aperture_size=None, grid="odak"):
+                 aperture_size=None, grid="odak", aperture=None):
+        """``aperture`` (optional): a custom Fourier aperture [2 pH, 2 pW] on the centred
+        grid (the propagator's zero-padded ``set_aperture`` tensor), used by the full-plane
+        ``reconstruct_planes`` in place of the circular ``aperture_size``."""
         self.W, self.H = int(img_size[0]), int(img_size[1])
         self.pH, self.pW = int(pad_size[0]), int(pad_size[1])
         self.wavelengths = [float(w) for w in wavelengths]
@@ -25,6 +28,13 @@ class HologramOptics:
         self.pitch = float(pixel_pitch)
         self.aperture = float(aperture_size) if aperture_size is not None else float(max(self.pH, self.pW))
         self.grid = {"odak": 0, "fft": 1}[grid]
+        self.custom_aperture = None
+        if aperture is not None:
+            a = torch.as_tensor(aperture, dtype=torch.float32).contiguous()
+            if tuple(a.shape) != (2 * self.pH, 2 * self.pW):
+                raise ValueError(f"aperture must be {2 * self.pH}x{2 * self.pW}, got "
+                                 f"{tuple(a.shape)}")
+            self.custom_aperture = a
         self._ws = {}
 
     def _workspace(self, device, full_plane=0):
@@ -70,6 +80,18 @@ class HologramOptics:
         return (self.C, self.pH, self.pW, self.D, _lib.doubles(self.wavelengths),
                 _lib.doubles(self.distances), self.pitch, self.aperture, self.grid)
 
+    def _aperture_on(self, device):
+        """The custom aperture on ``device`` (cached), or None."""
+        if self.custom_aperture is None:
+            return None
+        if self.custom_aperture.device != torch.device(device):
+            self.custom_aperture = self.custom_aperture.to(device)
+        return self.custom_aperture
+
+    def _aperture_common(self):
+        return (self.C, self.pH, self.pW, self.D, _lib.doubles(self.wavelengths),
+                _lib.doubles(self.distances), self.pitch, self.grid)
+
     def reconstruct_planes(self, phase, amplitude, keep_field=False):
         """propagator.reconstruct for one frame per channel over the full padded plane:
         phase, amplitude [C, pH, pW] -> intensity [D, C, pH, pW], and with ``keep_field``
@@ -79,11 +101,17 @@ class HologramOptics:
         inten = torch.empty(self.D, self.C, self.pH, self.pW, device=dev)
         out = torch.empty(self.D, self.C, self.pH, self.pW, dtype=torch.complex64, device=dev) \
             if keep_field else None
-        _lib.check(_lib.lib().cvqpg_blasm_reconstruct(
-            _lib.ptr(phase.contiguous().float()), _lib.ptr(amplitude.contiguous().float()),
-            *self._plane_common(), _lib.ptr(ws), ws.numel(),
-            None if out is None else _lib.ptr(torch.view_as_real(out)), _lib.ptr(inten),
-            _lib.stream_handle(dev)))
+        ph, am = phase.contiguous().float(), amplitude.contiguous().float()
+        of = None if out is None else _lib.ptr(torch.view_as_real(out))
+        ap = self._aperture_on(dev)
+        if ap is None:
+            _lib.check(_lib.lib().cvqpg_blasm_reconstruct(
+                _lib.ptr(ph), _lib.ptr(am), *self._plane_common(), _lib.ptr(ws), ws.numel(), of,
+                _lib.ptr(inten), _lib.stream_handle(dev)))
+        else:
+            _lib.check(_lib.lib().cvqpg_blasm_reconstruct_aperture(
+                _lib.ptr(ph), _lib.ptr(am), _lib.ptr(ap), *self._aperture_common(), _lib.ptr(ws),
+                ws.numel(), of, _lib.ptr(inten), _lib.stream_handle(dev)))
         return (inten, out) if keep_field else inten
 
     def reconstruct_planes_backward(self, phase, amplitude, grad_intensity, out_field):
@@ -92,11 +120,19 @@ class HologramOptics:
         ws = self._workspace(dev, 1)
         gp = torch.empty(self.C, self.pH, self.pW, device=dev)
         ga = torch.empty_like(gp)
-        _lib.check(_lib.lib().cvqpg_blasm_reconstruct_backward(
-            _lib.ptr(phase.contiguous().float()), _lib.ptr(amplitude.contiguous().float()),
-            _lib.ptr(grad_intensity.contiguous().float()),
-            _lib.ptr(torch.view_as_real(out_field)), *self._plane_common(), _lib.ptr(ws),
-            ws.numel(), _lib.ptr(gp), _lib.ptr(ga), _lib.stream_handle(dev)))
+        ph, am = phase.contiguous().float(), amplitude.contiguous().float()
+        gi = grad_intensity.contiguous().float()
+        of = _lib.ptr(torch.view_as_real(out_field))
+        ap = self._aperture_on(dev)
+        if ap is None:
+            _lib.check(_lib.lib().cvqpg_blasm_reconstruct_backward(
+                _lib.ptr(ph), _lib.ptr(am), _lib.ptr(gi), of, *self._plane_common(),
+                _lib.ptr(ws), ws.numel(), _lib.ptr(gp), _lib.ptr(ga), _lib.stream_handle(dev)))
+        else:
+            _lib.check(_lib.lib().cvqpg_blasm_reconstruct_aperture_backward(
+                _lib.ptr(ph), _lib.ptr(am), _lib.ptr(ap), _lib.ptr(gi), of,
+                *self._aperture_common(), _lib.ptr(ws), ws.numel(), _lib.ptr(gp), _lib.ptr(ga),
+                _lib.stream_handle(dev)))
         return gp, ga
 
 

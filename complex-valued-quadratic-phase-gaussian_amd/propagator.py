@@ -16,8 +16,9 @@ so ``no_grad=False`` trains through it as train_2d_gaussian_rgb.py:94 does) and 
 torch.fft with a cached transfer function when ``backend='torch'`` (the reference's own
 structure, kept as the PyTorch baseline).  There is no fallback between the two: the
 HIP backend raises if the library is missing.  A custom aperture tensor
-(utils/propagator.py:195-200, zero-padded to the 2x grid) is supported by the torch
-backend; the HIP kernels take circular apertures only and reject it.
+(utils/propagator.py:195-200, zero-padded to the 2x grid) runs on both backends (on HIP
+through cvqpg_blasm_reconstruct_aperture, which applies it, as custom() does, to the
+spectrum and to the transfer function).
 """
 import math
 
@@ -108,9 +109,6 @@ class propagator:  # noqa: N801  (reference class name)
         self.aperture_is_open = bool(self.aperture.min() >= 1.0)
         if backend not in ("hip", "torch"):
             raise ValueError(f"backend must be 'hip' or 'torch', got {backend!r}")
-        if backend == "hip" and self.custom_aperture:
-            raise NotImplementedError("the HIP BLASM kernels take circular apertures only; "
-                                      "use backend='torch' for a custom aperture tensor")
         self._kernels = None
         self._optics = None
 
@@ -155,7 +153,8 @@ class propagator:  # noqa: N801  (reference class name)
             pH, pW = self.resolution
             self._optics = optics_mod.HologramOptics(
                 (pW, pH), (pH, pW), self.wavelengths, self.pixel_pitch, self.distance_list(),
-                aperture_size=self.aperture_size, grid=self.grid)
+                aperture_size=self.aperture_size, grid=self.grid,
+                aperture=self.aperture if self.custom_aperture else None)
         return self._optics
 
     def _reconstruct_hip(self, phases, amplitude):

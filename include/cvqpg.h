@@ -264,6 +264,29 @@ int cvqpg_blasm_reconstruct_backward(const float *phase, const float *amplitude,
                                      void *ws, size_t ws_bytes, float *grad_phase,
                                      float *grad_amplitude, void *stream);
 
+/* cvqpg_blasm_reconstruct / _backward with a custom Fourier aperture instead of the circular
+ * one (propagator(aperture=...), utils/propagator.py:195-200 and custom() :41-64): `aperture`
+ * is a DEVICE array [2 pad_height][2 pad_width] float32 on the centred frequency grid (the
+ * hologram-resolution aperture zero-padded to the 2x grid, as the reference's set_aperture
+ * builds it); custom() applies it to the spectrum and to the transfer function, i.e. a
+ * factor aperture^2 on every plane. */
+int cvqpg_blasm_reconstruct_aperture(const float *phase, const float *amplitude,
+                                     const float *aperture, int channels, int pad_height,
+                                     int pad_width, int planes, const double *wavelengths,
+                                     const double *distances, double pixel_pitch, int grid,
+                                     void *ws, size_t ws_bytes, float *out_field,
+                                     float *intensity, void *stream);
+int cvqpg_blasm_reconstruct_aperture_backward(const float *phase, const float *amplitude,
+                                              const float *aperture,
+                                              const float *grad_intensity,
+                                              const float *out_field, int channels,
+                                              int pad_height, int pad_width, int planes,
+                                              const double *wavelengths,
+                                              const double *distances, double pixel_pitch,
+                                              int grid, void *ws, size_t ws_bytes,
+                                              float *grad_phase, float *grad_amplitude,
+                                              void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Multi-plane loss.  Replaces the PyTorch loss assembly of                  */
 /* train_2d_gaussian_rgb.py:191-197: multiplane_loss_odak.__call__           */

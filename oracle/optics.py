@@ -96,15 +96,21 @@ class BLASM:
     """
 
     def __init__(self, pad_size, wavelengths, pixel_pitch, distances, aperture_size=None,
-                 grid="odak", phase_dtype=np.float64):
+                 grid="odak", phase_dtype=np.float64, aperture=None):
+        """``aperture`` (optional, [pH, pW]): a custom Fourier aperture at the hologram's
+        resolution, zero-padded to the 2x grid as set_aperture does
+        (utils/propagator.py:199-200); otherwise the circular mask of ``aperture_size``."""
         self.pH, self.pW = pad_size
         self.wavelengths = list(wavelengths)
         self.dx = float(pixel_pitch)
         self.distances = [float(z) for z in distances]
         nu, nv = 2 * self.pH, 2 * self.pW
-        if aperture_size is None:
-            aperture_size = max(self.pH, self.pW)
-        self.aperture = torch.as_tensor(circular_mask(nu, nv, aperture_size))
+        if aperture is not None:
+            self.aperture = zero_pad2(torch.as_tensor(aperture, dtype=torch.float64))
+        else:
+            if aperture_size is None:
+                aperture_size = max(self.pH, self.pW)
+            self.aperture = torch.as_tensor(circular_mask(nu, nv, aperture_size))
         self.H = torch.stack([torch.stack([
             torch.as_tensor(transfer_function(nu, nv, self.dx, wl, z, grid, phase_dtype))
             for wl in self.wavelengths]) for z in self.distances])           # [D, C, nu, nv]

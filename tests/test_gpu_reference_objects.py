@@ -101,25 +101,36 @@ def test_reconstruct_hip_general_channel_power(pkg):
     assert e <= 1e-5
 
 
-def test_custom_aperture_torch_backend_and_hip_rejects(pkg):
+def test_custom_aperture_on_both_backends(pkg):
     """A custom aperture tensor at the hologram resolution, zero-padded to the 2x grid and
-    applied to the spectrum and to H (utils/propagator.py:59-61,199-200)."""
-    pH, pW, zs = 64, 96, [1e-3]
-    ph, am = phases_amps(1, pH, pW, 5)
-    rng = np.random.default_rng(6)
-    ap = torch.as_tensor(rng.uniform(0.3, 1.0, (pH, pW)), dtype=torch.float32)
+    applied to the spectrum and to H (utils/propagator.py:59-61,199-200): the torch backend
+    and the HIP kernels (cvqpg_blasm_reconstruct_aperture) against the FP64 oracle, on a
+    runtime-plan grid and on the 1080p guard band's 2816-row grid (compile-time column
+    plans, row-pair adjoint layout), intensities and, through no_grad=False, the phase and
+    amplitude gradients."""
     from cvqpg_amd.propagator import propagator
-    kw = dict(resolution=(pH, pW), wavelengths=[515e-9], pixel_pitch=8e-6, number_of_frames=1,
-              distances=zs, device=DEV, aperture=ap)
-    got = propagator(backend="torch", **kw).reconstruct(ph.to(DEV), am.to(DEV))
-    bl = ooptics.BLASM((pH, pW), [515e-9], 8e-6, zs, aperture_size=1e9)
-    bl.aperture = ooptics.zero_pad2(ap.double()[None])[0]
-    ref = bl.reconstruct(torch.polar(am.double(), ph.double()))
-    e = rel(got[0], ref)
-    print(f"custom aperture, torch backend vs FP64 {e:.1e}", flush=True)
-    assert e <= 1e-5
-    with pytest.raises(NotImplementedError):
-        propagator(backend="hip", **kw)
+    for pH, pW, zs in ((64, 96, [1e-3, 4e-3]), (1408, 160, [2e-3])):
+        ph, am = phases_amps(1, pH, pW, 5)
+        rng = np.random.default_rng(6)
+        ap = torch.as_tensor(rng.uniform(0.3, 1.0, (pH, pW)), dtype=torch.float32)
+        ap[: pH // 4] = 0.0                          # a blocked band
+        kw = dict(resolution=(pH, pW), wavelengths=[515e-9], pixel_pitch=8e-6,
+                  number_of_frames=1, distances=zs, device=DEV, aperture=ap)
+        bl = ooptics.BLASM((pH, pW), [515e-9], 8e-6, zs, aperture=ap.double())
+        ph64, am64 = ph.double().requires_grad_(), am.double().requires_grad_()
+        ref = bl.reconstruct(torch.polar(am64, ph64))               # [D, 1, pH, pW]
+        wgt = torch.as_tensor(rng.normal(size=ref.shape))
+        (ref * wgt).sum().backward()
+        got_t = propagator(backend="torch", **kw).reconstruct(ph.to(DEV), am.to(DEV))
+        php, amp = ph.to(DEV).requires_grad_(), am.to(DEV).requires_grad_()
+        got_h = propagator(backend="hip", **kw).reconstruct(php, amp, no_grad=False)
+        (got_h[0] * wgt.to(DEV, torch.float32)).sum().backward()
+        et, eh = rel(got_t[0], ref), rel(got_h[0], ref)
+        eg_p, eg_a = rel(php.grad, ph64.grad), rel(amp.grad, am64.grad)
+        print(f"custom aperture {pH}x{pW}: torch backend {et:.1e}, HIP {eh:.1e} vs FP64; "
+              f"HIP gradients phase {eg_p:.1e}, amplitude {eg_a:.1e}", flush=True)
+        assert et <= 1e-5 and eh <= 1e-5
+        assert eg_p <= 2e-5 and eg_a <= 2e-5
 
 
 @pytest.mark.parametrize("noise", [False, True])

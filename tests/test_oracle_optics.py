@@ -155,3 +155,32 @@ def test_circular_mask_matches_loop():
             x = -px / 2 + i * px / (px - 1)
             y = -py / 2 + j * py / (py - 1)
             assert m[i, j] == (1.0 if math.hypot(x, y) < r else 0.0)
+
+
+def test_custom_aperture_scales_each_bin_by_its_square():
+    """A custom aperture a [pH, pW] (utils/propagator.py:199-200: zero-padded to the 2x grid;
+    custom() multiplies the shifted spectrum and H by it): a plane wave on FFT bin (p, q)
+    comes out multiplied by a_pad(p, q)^2 H(f), a_pad read at the bin's centred position
+    (p + nu/2 mod nu), so bins outside the padded support vanish."""
+    pH, pW, z = 16, 24, 2e-3
+    nu, nv = 2 * pH, 2 * pW
+    rng = np.random.default_rng(11)
+    a = rng.uniform(0.3, 1.2, (pH, pW))
+    prop = optics.BLASM((pH, pW), [LAM], DX, [z], grid="fft", aperture=a)
+    a_pad = np.zeros((nu, nv))
+    a_pad[pH - pH // 2:pH - pH // 2 + pH, pW - pW // 2:pW - pW // 2 + pW] = a
+    yy, xx = np.mgrid[0:pH, 0:pW]
+    fr, fc = np.fft.fftfreq(nu, DX), np.fft.fftfreq(nv, DX)
+    for p, q in [(0, 0), (3, -5), (-7, 11), (15, 2), (-16, -24), (9, 13)]:
+        # the plane wave restricted to the hologram (what field_at pads and propagates) is
+        # not a single bin of the 2x grid; use the full-grid wave and the uncropped pipeline
+        U = torch.as_tensor(np.exp(2j * np.pi * (p * np.arange(nu)[:, None] / nu +
+                                                 q * np.arange(nv)[None, :] / nv)))
+        ap = prop.aperture.to(U.dtype)
+        S = torch.fft.fftshift(torch.fft.fft2(U)) * ap
+        out = torch.fft.ifft2(torch.fft.ifftshift((prop.H[0, 0] * ap) * S))
+        ci, cj = (p % nu + nu // 2) % nu, (q % nv + nv // 2) % nv
+        f2 = fr[p % nu] ** 2 + fc[q % nv] ** 2
+        expect = a_pad[ci, cj] ** 2 * np.exp(2j * np.pi * z * np.sqrt(1 / LAM ** 2 - f2))
+        np.testing.assert_allclose(out.numpy(), expect * U.numpy(), rtol=0, atol=1e-10)
+    assert np.isclose(float(prop.aperture.sum()), a.sum())
