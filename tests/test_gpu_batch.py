@@ -4,7 +4,9 @@ The batched step runs every scene's own kernels, except that the raster forward 
 adjoint of all scenes come from one persistent launch draining a shared tile queue.
 Each tile is computed by the same per-tile code in the same order as in the scene's own
 launch, so the bar is bit-identity with the scenes' single-scene fused steps (whose
-parity with the FP64 oracle is tested in test_gpu_step.py).
+parity with the FP64 oracle is tested in test_gpu_step.py), for the small scenes below and
+for all 16 scenes of the 16-scene configuration; each scene of a batched step is also
+compared with the FP64 oracle step directly.
 """
 import pytest
 import torch
@@ -121,6 +123,68 @@ def test_full_size_scenes_bit_identical(pkg):
         assert same(a.loss, b.loss)
         for p, q in zip(a.params, b.params):
             assert same(p, q)
+
+
+def test_batched_step_matches_fp64_oracle(pkg):
+    """Each scene of one batched step (three uneven scenes, the raster forward and adjoint of
+    all of them in one persistent launch each) against the FP64 oracle step directly, with the
+    bars of the single-scene step test: loss <= 2e-6 relative, raw gradients <= 3e-5
+    relative L2 per tensor."""
+    from oracle import adan as oadan
+    from oracle import step as ostep
+    from cvqpg_amd import fused
+    from cvqpg_amd.batch import BatchTrainer
+    wls = scenes(SMALL)
+    raw0 = [{n: getattr(wl.gaussians, n).detach().cpu().double().clone()
+             for n in ostep.adan.PARAM_NAMES} for wl in wls]
+    trs = [fused.FusedTrainer(wl, num_itrs=100, warmup_iters=0, inject_noise=False)
+           for wl in wls]
+    bt = BatchTrainer(None, trainers=trs, streams=2, chunk=2)
+    bt.run()
+    torch.cuda.synchronize()
+    for wl, tr, raw in zip(wls, trs, raw0):
+        ap = wl.args_prop
+        cfg = ostep.StepConfig(wl.img_size, ap.wavelengths, ap.pixel_pitch, ap.pad_size,
+                               ap.distances, aperture_size=ap.aperture_size, grid="odak")
+        L, grads, _ = ostep.loss_and_grads(raw, cfg, wl.targets.cpu().double(),
+                                           wl.masks.cpu().double())
+        grads, _ = oadan.clip_grads(grads, max_norm=float("inf"))
+        got = {n: g.detach().cpu().double() for n, g in tr.grad_views().items()}
+        el = abs(float(tr.loss.item()) - float(L)) / abs(float(L))
+        errs = {n: float((got[n] - grads[n]).norm() / grads[n].norm()) for n in grads}
+        print(f"{tr.W}x{tr.H} N={tr.N}: loss rel {el:.1e}; grads "
+              + " ".join(f"{n}={e:.1e}" for n, e in errs.items()), flush=True)
+        assert el <= 2e-6
+        for n, e in errs.items():
+            assert e <= 3e-5, (tr.W, tr.H, n, e)
+
+
+def test_all_sixteen_scenes_bit_identical(pkg):
+    """The whole 16-scene configuration (512^2, 1024x768, 1280x720, 1920x1080; 20k-150k
+    Gaussians, 1.07 M in all) as one captured batched step against the 16 scenes' own
+    single-scene fused steps: every loss and every parameter bit-identical after an eager
+    and a replayed step."""
+    from cvqpg_amd import synthetic
+    from cvqpg_amd.batch import BatchTrainer
+    specs = synthetic.batch_specs(16, seed=0)
+    ref = trainers(specs)
+    bt = BatchTrainer(None, trainers=trainers(specs), streams=4)
+    bt.run()
+    for t in ref:
+        t.run()
+    bt.capture()
+    bt.run()
+    for t in ref:
+        t.run()
+    torch.cuda.synchronize()
+    sizes = sorted({(a.W, a.H) for a in bt.trainers})
+    assert sizes == [(512, 512), (1024, 768), (1280, 720), (1920, 1080)]
+    for a, b in zip(bt.trainers, ref):
+        assert same(a.loss, b.loss), (a.W, a.H)
+        for p, q in zip(a.params, b.params):
+            assert same(p, q), (a.W, a.H)
+    print(f"16 scenes, {sum(t.N for t in bt.trainers)} Gaussians: all bit-identical; losses "
+          + " ".join(f"{float(t.loss):.4f}" for t in bt.trainers), flush=True)
 
 
 def test_autotune_restores_state(pkg):
