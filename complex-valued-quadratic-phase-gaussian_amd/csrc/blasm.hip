@@ -804,13 +804,18 @@ __global__ __launch_bounds__(P::nt) void k_a3_rows_fx(const float* __restrict__ 
 // in Z, so a wave's stores are contiguous); otherwise one after the other on nt threads,
 // the first row's spectrum kept in registers and each column pair stored as one 16-byte
 // piece.  grid (oh / 2, D*C)
+#ifndef CVQPG_A3_PAIR_THREADS
+#define CVQPG_A3_PAIR_THREADS 2   // threads of a two-row workgroup, in units of the plan's nt
+#endif
 template <class P>
 constexpr bool a3_pair_together() {
-  return 2 * P::nt <= 1024 &&
+  return CVQPG_A3_PAIR_THREADS * P::nt <= 1024 &&
          (P::Fwd::lds_elems(2) + fft::twiddle_entries(P::n)) * sizeof(float2) <= 80 * 1024;
 }
 template <class P>
-constexpr int a3_pair_threads() { return a3_pair_together<P>() ? 2 * P::nt : P::nt; }
+constexpr int a3_pair_threads() {
+  return a3_pair_together<P>() ? CVQPG_A3_PAIR_THREADS * P::nt : P::nt;
+}
 template <class P>
 constexpr int a3_pair_lds() { return a3_pair_together<P>() ? P::Fwd::lds_elems(2) : P::lds; }
 
@@ -823,7 +828,7 @@ __global__ __launch_bounds__(a3_pair_threads<P>()) void k_a3_rows_pair_fx(
   const float2* stw = stage_twiddles(plan, buf + a3_pair_lds<P>());
   const int r = 2 * blockIdx.x, dc = blockIdx.y;
   if constexpr (a3_pair_together<P>()) {
-    constexpr int NT = 2 * P::nt;
+    constexpr int NT = a3_pair_threads<P>();
     using GL = typename Fw::template GL<2, NT>;
     float2 v[GL::MAXB][Fw::LAST];
     a3_rows_fx<P, 2, NT>(grad_inten, out, o, plan, stw, buf, r, dc, v);
@@ -1025,7 +1030,13 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
     Iv::template middle<-1, CG, NT, true>(buf, stw, plan.nh, tt);
     float2 u[IL::MAXB][Iv::LAST];
     Iv::template last<-1, CG, NT, true>(buf, stw, plan.nh, u, tt);
-    float2* dst = Y + ((size_t)d * o.C + c) * o.oh * o.nv + col0;
+    // 32-bit element offsets from Y + col0 that include the plane: the eight row offsets of
+    // a thread are then not loop-invariant 64-bit addresses the compiler keeps across the
+    // plane loop (they spilled, and reloading a spilled address before a store makes the
+    // wave wait for every memory operation in flight, the previous plane's stores
+    // included).  Y holds D C oh nv < 2^32 elements (planes <= 16, oh <= 3072, nv <= 8704).
+    const unsigned pe = (unsigned)((d * o.C + c) * o.oh) * (unsigned)o.nv;
+    float2* Yc = Y + col0;
 #pragma unroll
     for (int b = 0; b < IL::MAXB; ++b) {
       if (!IL::live(b)) continue;
@@ -1034,7 +1045,7 @@ __global__ __launch_bounds__(P::nt, P::wpe_f) void k_f2_cols_fx(const float2* __
 #pragma unroll
       for (int k = Iv::LAST / 4; k < 3 * Iv::LAST / 4; ++k) {
         const int rr = j + k * IL::CNT - o.or0;
-        if ((unsigned)rr < (unsigned)o.oh) dst[(unsigned)(rr * o.nv + col)] = u[b][k];
+        if ((unsigned)rr < (unsigned)o.oh) Yc[pe + (unsigned)(rr * o.nv + col)] = u[b][k];
       }
     }
   }
