@@ -211,6 +211,12 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
     for (int m = 0; m < 3; ++m) gring[k][m] = 0.f;
   float z4[4] = {0.f, 0.f, 0.f, 0.f};
   float ssum = 0.f, msum = 0.f;
+  // The gradient of a pixel row is stored at the start of the next step, before that step
+  // issues its loads: the loads of a step are waited for at the next step, and a store
+  // issued after them would be waited for with them (the memory counter is in order), so
+  // every step would wait out a store's round trip.
+  float gpend = 0.f;
+  int ipend = -1;
   load_row(ys);
   load_stats(ys - (WIN - 1));
 
@@ -237,6 +243,8 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       }
       const float cM = qM, cN = qN;
       const float2 cS = sN;
+      if (ipend >= 0) grp[ipend] = gpend;
+      ipend = -1;
       load_row(y + 1);
       load_pix(y - (WIN - 1) + 1);
       load_stats(y - (WIN - 1) + 1);
@@ -344,9 +352,11 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
         gM = 2.f * P.inv_n * (e1 + wgt * e2);
       }
       const float gS = gA + 2.f * I * gB + cT * gC;
-      grp[pr * Wd + mc] = (cI >= 0.f && cI <= 1.f) ? gM + gS : 0.f;
+      gpend = (cI >= 0.f && cI <= 1.f) ? gM + gS : 0.f;
+      ipend = pr * Wd + mc;
     }
   }
+  if (ipend >= 0) grp[ipend] = gpend;
   red[0][i] = ssum;
   red[1][i] = msum;
   __syncthreads();
