@@ -1385,10 +1385,15 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   const int bits = tile_bits(T);
   const int nb = sort_blocks(cap);
   uint32_t *kin = w.keys_a, *vin = w.vals_a, *kout = w.keys_b, *vout = w.vals_b;
-  for (int shift = 0; shift < bits; shift += 8) {
-    const int nbits = bits - shift < 8 ? bits - shift : 8;
+  // as few passes of at most 8 bits as the key needs, with the bits spread evenly over them
+  // (13 bits: 7 + 6 rather than 8 + 5): fewer buckets per pass make longer runs of each
+  // bucket per block, so the scatter's stores coalesce better
+  const int npass = (bits + 7) / 8;
+  const int pbits = npass > 0 ? (bits + npass - 1) / npass : 0;
+  for (int shift = 0; shift < bits; shift += pbits) {
+    const int nbits = bits - shift < pbits ? bits - shift : pbits;
     // the last pass scatters the Gaussian ids straight into point_list
-    if (shift + 8 >= bits) vout = reinterpret_cast<uint32_t*>(point_list);
+    if (shift + pbits >= bits) vout = reinterpret_cast<uint32_t*>(point_list);
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(SORT_THREADS), 0, s, kin, w.misc, shift,
                        (1u << nbits) - 1u, nb, w.hist);
     CVQPG_LAUNCH_CHECK();
