@@ -636,6 +636,25 @@ struct BwdLds {
   int hist[TILE + 1];
 };
 
+// Lanes that evaluate one key together in the raster adjoint (2, 4 or 8).
+#ifndef CVQPG_BWD_LANES
+#define CVQPG_BWD_LANES 4
+#endif
+constexpr int BWD_LANES = CVQPG_BWD_LANES;
+static_assert(BWD_LANES == 2 || BWD_LANES == 4 || BWD_LANES == 8, "2, 4 or 8 lanes per key");
+
+// Sum over the BWD_LANES consecutive lanes of a key's group, the same value in each lane,
+// in a fixed order ((v0 + v1) + (v2 + v3) for four lanes; for eight, the two quads' sums
+// added), through DPP permutations.  Every lane of the group must be active.
+__device__ __forceinline__ float group_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+  if (BWD_LANES >= 4)
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  if (BWD_LANES == 8)
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+  return v;
+}
+
 // Rows of the tile on which a Gaussian reaches 2^QSKIP: for each row, the maximum of the
 // exponent over the row's pixel span [x0, x1] (the concave quadratic's peak along the row,
 // clamped to the span).  The row maximum is concave in y, so the rows form one range;
@@ -661,12 +680,14 @@ __device__ __forceinline__ int row_span(float mx, float my, float a00, float a01
 // the tile adds less than 2^-24 of a unit term to every adjoint sum (the reference
 // evaluates it, down to its clamp at e^-50): such keys get a zero partial record (so the
 // per-Gaussian sums stay complete) and are not evaluated.  The others are queued with the
-// range of rows on which they reach 2^-24 and evaluated 256 at a time, one per lane, each
-// lane summing over its own rows (rows outside the range add less than 2^-24 of a unit
-// term, like the dropped keys).  Every round is first sorted by row count, so that the 64
-// lanes of a wave walk similar numbers of rows: at 1080p this evaluates about three
-// quarters of the rows of the queued keys.  A key's record depends only on the key, so
-// the order in which lanes take keys does not change any result.  The first write to the
+// range of rows on which they reach 2^-24 and evaluated in rounds of up to 256 keys, a
+// group of BWD_LANES lanes per key, each group summing over the key's rows (rows outside
+// the range add less than 2^-24 of a unit term, like the dropped keys).  Every round is
+// first sorted by row count, so that the keys of a wave walk similar numbers of rows (a
+// wave holds 64 / BWD_LANES keys, so the fewer lanes per key, the more the row counts
+// within a wave differ); at 1080p the queued keys' row ranges hold about three quarters
+// of their rows.  A key's record depends only on the key, so the order in which groups
+// take keys does not change any result.  The first write to the
 // pixel gradients is not preceded by a barrier here: a workgroup that calls this for
 // several tiles places one between the calls.
 template <int C>
@@ -700,54 +721,54 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
 #pragma unroll
   for (int c = 0; c < C; ++c) kc[c] = kvec[c];
 
-  auto process = [&](int g, int span, bool valid) {
+  // BWD_LANES lanes per key: lane cg of the key's group covers the NCOL = 16 / BWD_LANES
+  // columns NCOL cg .. NCOL cg + NCOL - 1 of the key's rows, and the group's sums are
+  // combined by group_sum (a fixed order).  Columns past the image's right edge hold zero
+  // pixel gradients, which add exact zeros to every sum.  Per column the lane keeps the
+  // moments T0 = sum h and T1 = sum h dy over the rows, and T2 = sum h dy^2 over its
+  // columns; the mean and inverse-covariance gradients are formed from them once per key.
+  constexpr int NCOL = TILE / BWD_LANES;
+  auto process = [&](int g, int span, bool valid, int cg) {
     const float4* src = packed + (size_t)g * (REC / 4);
     const float4 q0 = src[0], q1 = src[1], q2 = src[2], q3 = src[3];
     const float mx = q0.x, my = q0.y, a00 = q0.z, a01 = q0.w, a11 = q1.x;
     const float o = q1.y, beta = q1.z;
-    float u[C], w[C], col[C], kA[C];
+    float u[C], w[C], col[C], kA[C], colh[C];
     u[0] = q1.w; w[0] = q2.x; col[0] = q2.y;
     if (C == 3) {
       u[1] = q2.z; w[1] = q2.w; col[1] = q3.x;
       u[C - 1] = q3.y; w[C - 1] = q3.z; col[C - 1] = q3.w;
     }
-    float colh[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       kA[c] = kc[c] * col[c];
       colh[c] = -0.5f * col[c];
     }
-    // Per pixel the quadratic form is q = dx (a00 dx + 2 a01 dy) + a11 dy^2 (two FMAs
-    // from per-row constants), and the pixel sums that need dx or dy are kept as the
-    // row moments S0 = sum h, S1 = sum h dx, S2 = sum h dx^2 of h = dL/dq / o, folded
-    // into the mean and inverse-covariance gradients once per row.  The opacity factor
-    // of alpha = o G is applied once at the end, and dL/do = sum_c col_c dL/dcol_c / o
-    // is formed from the colour sums after the loop.  Pairs beyond the power clamp
-    // (q < log2(e^-50)) get G = 2^q instead of e^-50 and their exponent-path term is
-    // not masked: both change the pair's terms by less than e^-50 ~ 2e-22.
     float dcol[C], dph[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) dcol[c] = dph[c] = 0.f;
-    float K = 0.f, di00 = 0.f, di01 = 0.f, di11 = 0.f, dmx = 0.f, dmy = 0.f;
-    const float dx0 = mx - (float)x0;
+    float K = 0.f, T2 = 0.f, T0[NCOL], T1[NCOL];
+#pragma unroll
+    for (int j = 0; j < NCOL; ++j) T0[j] = T1[j] = 0.f;
+    const float dxc = mx - (float)(x0 + NCOL * cg);
     const int r0 = span & 0xff, nr = span >> 8;
-    // the round is sorted by row count, so the wave's first lane walks the most rows
+    // the round is sorted by row count, so the wave's first key walks the most rows
     const int nmax = __builtin_amdgcn_readfirstlane(nr);
     for (int k = 0; k < nmax; ++k) {
       if (k >= nr) continue;
       const int yy = r0 + k;
       const float dy = my - (float)(y0 + yy);
       const float bq = 2.0f * a01 * dy, cq = a11 * dy * dy;
-      float S0 = 0.f, S1 = 0.f, S2 = 0.f;
-      const float2* gr0 = &L.grad[0][yy * GROW];
-      auto pixel = [&](int xx) {
-        const float dx = dx0 - (float)xx;
+      const float2* gr0 = &L.grad[0][yy * GROW + NCOL * cg];
+#pragma unroll
+      for (int j = 0; j < NCOL; ++j) {
+        const float dx = dxc - (float)j;
         const float q = fmaf(dx, fmaf(a00, dx, bq), cq);
         const float G = fast_exp2(q);
         float Sh = 0.f, Tb = 0.f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float2 gg = gr0[c * TILE * GROW + xx];
+          const float2 gg = gr0[c * TILE * GROW + j];
           const float rev = fract_f(fmaf(w[c], q, u[c]));
           const float sn = sin_rev(rev), cs = cos_rev(rev);
           const float Pc = fmaf(gg.x, cs, gg.y * sn);
@@ -759,23 +780,34 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
         }
         const float h = G * fmaf(Tb, beta, Sh);
         K = fmaf(G * Tb, q, K);
-        S0 += h;
-        const float hd = h * dx;
-        S1 += hd;
-        S2 = fmaf(hd, dx, S2);
-      };
-      if (nx == TILE) {
-#pragma unroll
-        for (int xx = 0; xx < TILE; ++xx) pixel(xx);
-      } else {
-        for (int xx = 0; xx < nx; ++xx) pixel(xx);
+        T0[j] += h;
+        const float hy = h * dy;
+        T1[j] += hy;
+        T2 = fmaf(hy, dy, T2);
       }
-      di00 += S2;
-      di01 = fmaf(dy, S1, di01);
-      di11 = fmaf(dy * dy, S0, di11);
-      dmx += fmaf(a00, S1, a01 * dy * S0);
-      dmy += fmaf(a01, S1, a11 * dy * S0);
     }
+    // S1 = sum h dx, M00 = sum h dx^2, M01 = sum h dx dy, M1y = sum h dy over the lane
+    float S1 = 0.f, M00 = 0.f, M01 = 0.f, M1y = 0.f;
+#pragma unroll
+    for (int j = 0; j < NCOL; ++j) {
+      const float dx = dxc - (float)j;
+      const float t0 = T0[j] * dx;
+      S1 += t0;
+      M00 = fmaf(t0, dx, M00);
+      M01 = fmaf(T1[j], dx, M01);
+      M1y += T1[j];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      dcol[c] = group_sum(dcol[c]);
+      dph[c] = group_sum(dph[c]);
+    }
+    K = group_sum(K);
+    S1 = group_sum(S1);
+    M00 = group_sum(M00);
+    M01 = group_sum(M01);
+    M1y = group_sum(M1y);
+    T2 = group_sum(T2);
     float dop = 0.f;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -783,12 +815,6 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       dcol[c] *= o;
       dph[c] *= o;
     }
-    K *= o;
-    di00 *= o;
-    di01 *= o;
-    di11 *= o;
-    dmx *= o;
-    dmy *= o;
     if (valid) {
       const int4 rc = rect[g];
       const int e = rc.x + (ty - rc.z) * rc.w + (tx - rc.y);
@@ -799,18 +825,24 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
         rec[C + c] = dph[c];
       }
       rec[2 * C + 0] = dop;
-      rec[2 * C + 1] = K;
-      rec[2 * C + 2] = di00;
-      rec[2 * C + 3] = di01;
-      rec[2 * C + 4] = di11;
-      rec[2 * C + 5] = dmx;
-      rec[2 * C + 6] = dmy;
+      rec[2 * C + 1] = K * o;
+      rec[2 * C + 2] = M00 * o;
+      rec[2 * C + 3] = M01 * o;
+      rec[2 * C + 4] = T2 * o;
+      rec[2 * C + 5] = fmaf(a00, S1, a01 * M1y) * o;
+      rec[2 * C + 6] = fmaf(a01, S1, a11 * M1y) * o;
 #pragma unroll
       for (int i = 2 * C + 7; i < REC; ++i) rec[i] = 0.f;
-      float4* dst = partials + (size_t)e * (REC / 4);
+      // the group's lanes write the record's float4s (lane cg: numbers cg, cg + BWD_LANES,
+      // ...): one 64-byte run
 #pragma unroll
-      for (int i = 0; i < REC / 4; ++i)
-        dst[i] = make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
+      for (int i0 = 0; i0 < REC / 4; i0 += BWD_LANES) {
+        float4 v = make_float4(rec[4 * i0], rec[4 * i0 + 1], rec[4 * i0 + 2], rec[4 * i0 + 3]);
+#pragma unroll
+        for (int i = i0 + 1; i < i0 + BWD_LANES && i < REC / 4; ++i)
+          if (cg == i - i0) v = make_float4(rec[4 * i], rec[4 * i + 1], rec[4 * i + 2], rec[4 * i + 3]);
+        if (i0 + cg < REC / 4) partials[(size_t)e * (REC / 4) + i0 + cg] = v;
+      }
     }
   };
 
@@ -880,9 +912,13 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       }
       filled -= n;
       __syncthreads();
-      if (wave * 64 < n) {
-        const int slot = wave * 64 + lane;
-        process(L.rkeys[slot < n ? slot : 0], slot < n ? L.rspan[slot] : 0, slot < n);
+      // groups of KPW consecutive keys of the sorted round, BWD_LANES lanes per key; wave w
+      // takes the groups w, w + 4, w + 8, ... so that every wave gets long and short keys
+      constexpr int KPW = 64 / BWD_LANES;
+      for (int gi = wave; gi * KPW < n; gi += 4) {
+        const int slot = gi * KPW + lane / BWD_LANES;
+        process(L.rkeys[slot < n ? slot : 0], slot < n ? L.rspan[slot] : 0, slot < n,
+                lane % BWD_LANES);
       }
       __syncthreads();  // every lane is done with rkeys / rspan
     }
