@@ -26,6 +26,7 @@ SIGNATURES = {
     "cvqpg_abi_version": (_c_int, []),
     "cvqpg_raster_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_i64]),
     "cvqpg_bin_count": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _c_i64, _p, _p]),
+    "cvqpg_bin_count_live": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _c_i64, _p, _p]),
     "cvqpg_bin_sort": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _c_i64, _p, _p, _p]),
     "cvqpg_render_forward": (_c_int, [_p] * 8 + [_c_int, _c_int, _c_f32, _c_int, _c_int, _p,
                                                  _c_size, _c_i64, _p, _p, _p, _p]),

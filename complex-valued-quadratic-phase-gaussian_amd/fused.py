@@ -4,7 +4,7 @@ One iteration of train_2d_gaussian_rgb.py:187-211 as a fixed sequence of libcvqp
 on the current stream:
 
     prologue      cvqpg_prologue_forward       activations, cov, inverse, radii, packing
-    binning       cvqpg_bin_count, cvqpg_bin_sort
+    binning       cvqpg_bin_count_live, cvqpg_bin_sort
     raster        cvqpg_render_forward_packed  -> field [C,H,W]
     optics        cvqpg_blasm_forward          -> plane fields + intensities [D,C,H,W]
     loss          cvqpg_loss_forward_backward_stats  -> dL/dI, loss (target statistics
@@ -177,8 +177,10 @@ class FusedTrainer:
             _lib.ptr(self.means), _lib.ptr(self.radii), self._s()))
 
     def _count(self):
+        # the raster kernels' own tile lists (keys below their skip threshold left out; the
+        # step's results are those of the full lists bit for bit)
         ws, nb, cap = self._ws()
-        _lib.check(_lib.lib().cvqpg_bin_count(
+        _lib.check(_lib.lib().cvqpg_bin_count_live(
             _lib.ptr(self.means), _lib.ptr(self.radii), self.N, self.W, self.H, ws, nb, cap,
             _lib.ptr(self.num_pairs), self._s()))
 

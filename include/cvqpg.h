@@ -53,6 +53,21 @@ int cvqpg_bin_count(const float *means, const int32_t *radii, int n, int width, 
                     void *ws, size_t ws_bytes, int64_t capacity, int32_t *num_pairs,
                     void *stream);
 
+/* cvqpg_bin_count for the raster kernels' own tile lists.  No counterpart in the reference,
+ * which keys every tile of the 5-sigma rectangle (cuda/forward.cu:15-26); this counts only
+ * the tiles on which the Gaussian's exponent reaches the raster kernels' skip threshold
+ * (2^-24 of a unit term) less a small margin, read from the packed records that
+ * cvqpg_prologue_forward wrote into ws (call it first).  The forward and adjoint tile
+ * kernels skip every other key (the adjoint writes it a zero partial record), so their
+ * results are bit for bit those of the full lists; the lists are shorter (about 63 % of
+ * the keys at 1920x1080, 150k Gaussians).  Rectangles of more than 64 tiles keep every
+ * tile.  The following cvqpg_bin_sort then writes tile lists that are subsequences of the
+ * reference's; the kept tiles stay in ws (a flag there marks a live count), where
+ * cvqpg_render_backward_tiles and the batched adjoint find each key's partial-record slot. */
+int cvqpg_bin_count_live(const float *means, const int32_t *radii, int n, int width,
+                         int height, void *ws, size_t ws_bytes, int64_t capacity,
+                         int32_t *num_pairs, void *stream);
+
 /* Key-overflow flags of a raster workspace, for callers that size the key buffers once and
  * replay a captured step (the reference reads P back on every call and allocates to fit,
  * cuda/raster.cu:71-80, so it has no counterpart).  Host-only: *flags receives the DEVICE

@@ -58,6 +58,8 @@ def test_workspace_size_and_argument_errors(pkg):
     assert b"1 or 3 channels" in L.cvqpg_last_error()
     rc = L.cvqpg_bin_count(None, None, 10, 64, 64, None, 0, 0, None, None)
     assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+    rc = L.cvqpg_bin_count_live(None, None, 10, 64, 64, None, 0, 0, None, None)
+    assert rc == 1 and b"workspace" in L.cvqpg_last_error()
     # target window statistics: (D C) x (H - 10) x (W - 10) float2, checked before any launch
     assert L.cvqpg_loss_target_stats_size(3, 4, 40, 56) == 4 * 3 * 30 * 46 * 8
     assert L.cvqpg_loss_target_stats_size(3, 4, 10, 56) == 0
@@ -101,7 +103,7 @@ def test_batch_table_layout_and_errors(pkg):
         s.point_list, s.ranges, s.field = s.ws + 0x1000_0000, s.ws + 0x2000_0000, s.ws + 0x3000_0000
         s.grad_field, s.k = s.ws + 0x4000_0000, s.ws + 0x5000_0000
     nb = L.cvqpg_raster_batch_table_size(3)
-    assert nb == 64 + 3 * 88
+    assert nb == 64 + 3 * 96
     buf = (ctypes.c_uint8 * nb)()
     _lib.check(L.cvqpg_raster_batch_table(scenes, 3, buf, nb))
     raw = bytes(buf)
@@ -111,13 +113,14 @@ def test_batch_table_layout_and_errors(pkg):
     assert hdr[4] == 3 and hdr[5] == sum(tiles)
     begin = 0
     for i, (W, H, n) in enumerate(sizes):
-        rec = raw[64 + 88 * i: 64 + 88 * (i + 1)]
-        ptrs = np.frombuffer(rec[:72], dtype=np.uint64)
-        ints = np.frombuffer(rec[72:], dtype=np.int32)
+        rec = raw[64 + 96 * i: 64 + 96 * (i + 1)]
+        ptrs = np.frombuffer(rec[:80], dtype=np.uint64)
+        ints = np.frombuffer(rec[80:], dtype=np.int32)
         s = scenes[i]
-        # ranges, point_list, packed (= workspace base), rect, misc, field, grad, k, partials
+        # ranges, point_list, packed (= workspace base), rect, misc, field, grad, k, partials,
+        # kept-tile masks of the live binning
         assert ptrs[0] == s.ranges and ptrs[1] == s.point_list and ptrs[2] == s.ws
-        assert s.ws < ptrs[3] < ptrs[4] < ptrs[8] < s.ws + s.ws_bytes
+        assert s.ws < ptrs[3] < ptrs[9] < ptrs[4] < ptrs[8] < s.ws + s.ws_bytes
         assert ptrs[5] == s.field and ptrs[6] == s.grad_field and ptrs[7] == s.k
         assert list(ints) == [W, H, (W + 15) // 16, begin]
         begin += tiles[i]
