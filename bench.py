@@ -335,7 +335,8 @@ def bench_batch(args, dev):
     k = {}
     for name in ("render_fwd", "render_bwd"):
         k[name] = bt._time(lambda: bt._batched(name), 10)
-    pairs = sum(t.last_pairs for t in bt.trainers) * 256
+    keys_ref = sum(t.reference_keys() for t in bt.trainers)
+    pairs = keys_ref * 256
     v, t = raster_ops_per_pair(3, True)
     achieved = pairs * (v + t) / (k["render_bwd"] * 1e-3) / 1e12
     roof = {"kernel": "k_render_bwd_batch (persistent tile-queue raster adjoint, all scenes)",
@@ -349,7 +350,8 @@ def bench_batch(args, dev):
            "data": "synthetic",
            "config": {"workload": "batch16_uneven_rgb", "scenes": [
                           [t.W, t.H, t.N] for t in bt.trainers],
-                      "planes": 3, "keys": sum(t.last_pairs for t in bt.trainers),
+                      "planes": 3, "keys": keys_ref,
+                      "keys_live": sum(t.last_pairs for t in bt.trainers),
                       "sequential_value": n * 1e3 / ms_seq, "sequential_ms_per_round": ms_seq,
                       "speedup_vs_sequential": ms_seq / ms,
                       "autotune": {"render_fwd": {"grid": bt.grid["render_fwd"],
@@ -459,7 +461,7 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         rasterizer.timer.enabled = False
-        P = rasterizer._capacity_hint.get((N, W, H), 0)
+        P = P_ref = rasterizer._capacity_hint.get((N, W, H), 0)   # the reference's lists
         kernel_ms = {"render_fwd": rasterizer.timer.mean_ms("k_render_fwd"),
                      "render_bwd": rasterizer.timer.mean_ms("k_render_bwd")}
         stages = kernel_ms
@@ -502,8 +504,10 @@ def main():
             raise RuntimeError("key capacity exceeded during the timed steps")
         P = trainer.last_pairs
         one_graph_ms = split_ms
+        # the algorithmic unit counts the reference's tile lists (the step bins live lists)
+        P_ref = trainer.reference_keys()
     ms = 1e3 * dt / args.steps
-    pairs = P * 256
+    pairs = P_ref * 256
     roofs = rooflines(kernel_ms, spec, pairs, args.workload)
     roof = max(roofs, key=lambda e: e["avg_ms"]) if roofs else None
     peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9   # the step's own peak
@@ -540,7 +544,8 @@ def main():
            "data": "synthetic",
            "config": {"workload": args.workload, "width": W, "height": H, "gaussians": N,
                       "channels": C, "planes": len(synthetic.distances_for(spec)),
-                      "fft_grid": [2 * pH, 2 * pW], "keys": P, "path": path,
+                      "fft_grid": [2 * pH, 2 * pW], "keys": P_ref, "keys_live": P,
+                      "path": path,
                       "peak_hbm_gb": peak_gb, "stage_ms_eager": stages,
                       "ms_per_step_all_kernels_split": one_graph_ms},
            "roofline": roof, "rooflines": roofs,

@@ -141,7 +141,23 @@ class FusedTrainer:
         self._count()
         P = int(self.num_pairs.item())
         self.last_pairs = P
-        self._alloc_raster(max(1024, int(P * headroom)))
+        # sized on the reference's key count, which bounds every live count of the same
+        # parameters: while a fit sharpens its Gaussians the live lists grow much faster
+        # than the rectangles (+17 % against +4 % over 400 steps of a 256 x 256 fit)
+        self._alloc_raster(max(1024, int(max(P, self.reference_keys()) * headroom)))
+
+    def reference_keys(self):
+        """Number of keys of the reference's tile lists (every tile of each 5-sigma
+        rectangle, cuda/forward.cu:15-26) for the Gaussians of the last prologue, counted
+        with cvqpg_bin_count into a scratch workspace (the step's own buffers and overflow
+        flags are untouched).  One host read."""
+        L = _lib.lib()
+        nb = workspace_bytes(self.N, self.C, self.W, self.H, 1)
+        ws = torch.empty(nb, dtype=torch.uint8, device=self.dev)
+        out = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        _lib.check(L.cvqpg_bin_count(_lib.ptr(self.means), _lib.ptr(self.radii), self.N, self.W,
+                                     self.H, _lib.ptr(ws), nb, 1, _lib.ptr(out), self._s()))
+        return int(out.item())
 
     def check_capacity(self):
         """One host read of the last key count and of the number of overflowed steps since
