@@ -34,8 +34,8 @@ namespace cvqpg {
 
 constexpr int TILE = 16;
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;  // 4096 elements per block
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;  // 1024 elements per block
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ROUNDS = 16;
 constexpr int SORT_BLOCK = SORT_THREADS * SORT_ROUNDS;  // 4096 keys per block
@@ -316,18 +316,24 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t m, uint32_t k) {
 // Gaussian by a binary search over the workgroup's offsets in LDS.  Each Gaussian's keys
 // run over its rectangle row by row, as in the reference.  After a live count (misc[3])
 // a Gaussian's keys are the kept tiles of its rectangle (the set bits of live[g], in the
-// same row-major order; rectangles of more than 64 tiles keep them all).
-// grid ceil(n / 256)
+// same row-major order; rectangles of more than 64 tiles keep them all).  The grid first
+// clears the tile ranges (untouched tiles keep (0, 0)) and sets point_list[0] = 0 (the
+// reference's list when no key exists).  grid ceil(n / 256)
 __global__ __launch_bounds__(256) void k_duplicate(const uint32_t* __restrict__ counts,
                                                    const uint32_t* __restrict__ offs,
                                                    int4* __restrict__ rect,
                                                    const uint64_t* __restrict__ live,
                                                    const uint32_t* __restrict__ misc, int n,
                                                    int gx, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
+                                                   uint32_t* __restrict__ vals,
+                                                   int32_t* __restrict__ ranges, int64_t nr,
+                                                   int32_t* __restrict__ point_list) {
   __shared__ uint32_t s_off[257];  // offsets relative to the range start; [256] = its end
   __shared__ int4 s_rc[256];
   __shared__ uint64_t s_live[256];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nr; i += (int64_t)gridDim.x * 256)
+    ranges[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) point_list[0] = 0;
   if (misc[1]) return;  // uniform: the keys overflowed the capacity
   const bool lv = misc[3] != 0u;
   const int t = threadIdx.x, g0 = blockIdx.x * 256, g = g0 + t;
@@ -1552,12 +1558,15 @@ extern "C" int cvqpg_bin_sort(const float* means, const int32_t* radii, int n, i
   const int gx = ceil_div(width, TILE), gy = ceil_div(height, TILE);
   const int T = gx * gy;
   const int64_t cap = capacity > 0 ? capacity : 1;
-  // untouched tiles keep (0, 0); point_list[0] = 0 (the reference's list when no key exists)
-  hipLaunchKernelGGL(k_clear_ranges, dim3(ceil_div(2 * (int64_t)T, 256)), dim3(256), 0, s, ranges,
-                     2 * (int64_t)T, point_list);
+  // untouched tiles keep (0, 0); point_list[0] = 0 (the reference's list when no key
+  // exists): k_duplicate clears them first, k_clear_ranges when there is no Gaussian
   if (n > 0)
     hipLaunchKernelGGL(k_duplicate, dim3(ceil_div(n, 256)), dim3(256), 0, s, w.counts, w.offs,
-                       w.rect, w.live, w.misc, n, gx, w.keys_a, w.vals_a);
+                       w.rect, w.live, w.misc, n, gx, w.keys_a, w.vals_a, ranges,
+                       2 * (int64_t)T, point_list);
+  else
+    hipLaunchKernelGGL(k_clear_ranges, dim3(ceil_div(2 * (int64_t)T, 256)), dim3(256), 0, s,
+                       ranges, 2 * (int64_t)T, point_list);
   CVQPG_LAUNCH_CHECK();
   const int bits = tile_bits(T);
   const int nb = sort_blocks(cap);
