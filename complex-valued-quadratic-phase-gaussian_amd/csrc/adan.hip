@@ -74,35 +74,62 @@ __global__ __launch_bounds__(256) void k_adan(AdanParams A, const float* __restr
   const double T = A.num_itrs > 1 ? A.num_itrs : 1;
   const float lr_means =
       (float)(A.eta_min + (A.lr0 - A.eta_min) * (1.0 + cos(3.141592653589793 * itr / T)) / 2.0);
-  // grid-stride: every workgroup repeats the (small) norm reduction above once
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int seg = 0;
+  // grid-stride in batches of ADAN_BATCH elements per thread (element k of a batch at
+  // i0 + k * stride): every load of a batch is issued before any of its arithmetic, so a
+  // thread waits out one memory latency per batch instead of one per element; every
+  // workgroup repeats the (small) norm reduction above once
+  constexpr int ADAN_BATCH = 8;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const float sbc3 = sqrtf(bc3);
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + t; i0 < total; i0 += ADAN_BATCH * stride) {
+    float g[ADAN_BATCH], pg[ADAN_BATCH], m0[ADAN_BATCH], d0[ADAN_BATCH], v0[ADAN_BATCH],
+        p0[ADAN_BATCH];
+    float* pp[ADAN_BATCH];
+    int sg[ADAN_BATCH];
 #pragma unroll
-    for (int k = 1; k < 7; ++k) seg += i >= A.off[k] ? 1 : 0;
-    bool first = step == 1;
-    if (seg == A.curv_seg) {
-      if (!A.curv_enabled || itr < A.curv_warmup) continue;  // no gradient: untouched
-      first = first || itr == A.curv_warmup;                 // state created at unfreeze
+    for (int k = 0; k < ADAN_BATCH; ++k) {
+      const int64_t i = i0 + k * stride;
+      const int64_t ii = i < total ? i : 0;
+      int seg = 0;
+#pragma unroll
+      for (int q = 1; q < 7; ++q) seg += ii >= A.off[q] ? 1 : 0;
+      sg[k] = seg;
+      pp[k] = A.param[seg] + (ii - A.off[seg]);
+      g[k] = grads[ii];
+      pg[k] = npg[ii];
+      m0[k] = m[ii];
+      d0[k] = dd[ii];
+      v0[k] = v[ii];
+      p0[k] = *pp[k];
     }
-    const float lr = seg == 0 ? lr_means : A.lr[seg];
-    const float g = grads[i] * s_coef;
-    const float prev = first ? -g : npg[i];
-    const float diff = prev + g;
-    const float mi = m[i] * A.b1 + (1.0f - A.b1) * g;
-    const float di = dd[i] * A.b2 + (1.0f - A.b2) * diff;
-    const float u = diff * A.b2 + g;
-    const float vi = v[i] * A.b3 + (1.0f - A.b3) * u * u;
-    const float denom = sqrtf(vi) / sqrtf(bc3) + A.eps;
-    float* p = A.param[seg] + (i - A.off[seg]);
-    float pv = *p;
-    pv -= (lr / bc1) * mi / denom;
-    pv -= (lr * A.b2 / bc2) * di / denom;
-    *p = pv;
-    m[i] = mi;
-    v[i] = vi;
-    dd[i] = di;
-    npg[i] = -g;
+#pragma unroll
+    for (int k = 0; k < ADAN_BATCH; ++k) {
+      const int64_t i = i0 + k * stride;
+      if (i >= total) continue;
+      const int seg = sg[k];
+      bool first = step == 1;
+      if (seg == A.curv_seg) {
+        if (!A.curv_enabled || itr < A.curv_warmup) continue;  // no gradient: untouched
+        first = first || itr == A.curv_warmup;                 // state created at unfreeze
+      }
+      const float lr = seg == 0 ? lr_means : A.lr[seg];
+      const float gi = g[k] * s_coef;
+      const float prev = first ? -gi : pg[k];
+      const float diff = prev + gi;
+      const float mi = m0[k] * A.b1 + (1.0f - A.b1) * gi;
+      const float di = d0[k] * A.b2 + (1.0f - A.b2) * diff;
+      const float u = diff * A.b2 + gi;
+      const float vi = v0[k] * A.b3 + (1.0f - A.b3) * u * u;
+      const float denom = sqrtf(vi) / sbc3 + A.eps;
+      float pv = p0[k];
+      pv -= (lr / bc1) * mi / denom;
+      pv -= (lr * A.b2 / bc2) * di / denom;
+      *pp[k] = pv;
+      m[i] = mi;
+      v[i] = vi;
+      dd[i] = di;
+      npg[i] = -gi;
+    }
   }
 }
 
