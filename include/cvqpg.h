@@ -87,7 +87,8 @@ int cvqpg_raster_clear_overflow(void *ws, size_t ws_bytes, int n, int64_t capaci
  * key sort bit for bit) and writes point_list [max(P,1)] (sorted Gaussian ids;
  * [0] when P == 0) and ranges [T, 2] ([start, end), (0, 0) for untouched tiles).
  * If P > capacity nothing is sorted, every range is (0, 0) and the caller must
- * retry with a larger workspace.
+ * retry with a larger workspace.  After cvqpg_bin_count_live the keys are the kept
+ * tiles only, so each tile's list is an ordered subsequence of the reference's.
  * Replaces duplicateWithKeysKernel, cub::DeviceRadixSort::SortPairs and
  * identifyTileRangesKernel (cuda/forward.cu:28-60, cuda/raster.cu:77-95). */
 int cvqpg_bin_sort(const float *means, const int32_t *radii, int n, int width, int height,
