@@ -86,6 +86,9 @@ SIGNATURES = {
     "cvqpg_ssim_term": (_c_int, [_p, _p, _c_int, _c_int, _c_int, _p, _c_size, _p, _p, _p]),
     "cvqpg_prologue_forward": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _c_f32, _c_f32,
                                         _c_int, _c_f32, _p, _c_size, _c_i64, _p, _p, _p]),
+    "cvqpg_prologue_forward_live": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _p, _c_f32, _c_f32,
+                                        _c_int, _c_f32, _p, _c_size, _c_i64, _p, _p, _p]),
+    "cvqpg_bin_scan": (_c_int, [_c_int, _p, _c_size, _c_i64, _p, _p]),
     "cvqpg_raster_backward_raw": (_c_int, [_p, _c_int, _c_int, _c_int, _c_int, _c_f32, _c_f32,
                                            _c_int, _c_f32, _p, _c_int, _p, _c_size, _c_i64, _p,
                                            _p, _p]),

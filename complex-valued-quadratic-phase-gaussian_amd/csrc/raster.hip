@@ -557,6 +557,59 @@ __device__ __forceinline__ float tile_qmax(float mx, float my, float a00, float 
 // grid ceil(n / 256)
 constexpr float LIVE_MARGIN = 0.0625f;
 
+// The live count of one Gaussian from its mean, radius and packed exponent coefficients:
+// the rectangle (lx, ly, width w), the kept tiles (bit mask) and their number.
+__device__ __forceinline__ void live_tiles(float mx, float my, float a00, float a01, float a11,
+                                           int r, int gx, int gy, int W, int H, int& lx,
+                                           int& ly, int& w, uint32_t& c, uint64_t& m) {
+  int hx = 0, hy = 0;
+  lx = ly = 0;
+  if (r > 0) get_rect(mx, my, r, gx, gy, lx, ly, hx, hy);
+  w = hx - lx;
+  const int area = (hx > lx && hy > ly) ? (hy - ly) * w : 0;
+  c = 0;
+  m = 0ull;
+  if (area > 64) {
+    c = (uint32_t)area;
+    m = ~0ull;
+    return;
+  }
+  if (area == 0) return;
+  const float beta = a00 * a11 - a01 * a01;
+  if (!(beta > 0.f) || !(a00 < 0.f) || !(a11 < 0.f) || !isfinite(mx) || !isfinite(my) ||
+      !isfinite(beta)) {
+    m = area == 64 ? ~0ull : (1ull << area) - 1ull;
+  } else {
+    // hardware reciprocals and square roots (1 ulp): far inside the widening below
+    const float thr = QSKIP - LIVE_MARGIN;
+    const float i00 = __builtin_amdgcn_rcpf(a00), ib = __builtin_amdgcn_rcpf(beta);
+    const float rx = -a01 * i00, ry = -a01 * __builtin_amdgcn_rcpf(a11), ia = -i00;
+    const float dxr = __builtin_amdgcn_sqrtf(fmaxf(thr * a11 * ib, 0.f));  // rightmost point
+    const float dyr = ry * dxr;
+    const float dymax = __builtin_amdgcn_sqrtf(fmaxf(a00 * thr * ib, 0.f));
+    const float c0 = a00 * thr;
+    for (int ty = ly; ty < hy; ++ty) {
+      const float dylo = my - (float)(min(ty * TILE + TILE, H) - 1), dyhi = my - (float)(ty * TILE);
+      if (dyhi < -dymax || dylo > dymax) continue;  // the row misses the ellipse
+      // dx = mx - x of the interval's ends: the extreme points if the row holds them,
+      // else the chord at the row edge nearest to them
+      const float er = fminf(fmaxf(dyr, dylo), dyhi), el = fminf(fmaxf(-dyr, dylo), dyhi);
+      const float dr = (dyr >= dylo && dyr <= dyhi)
+          ? dxr : fmaf(rx, er, __builtin_amdgcn_sqrtf(fmaxf(c0 - beta * er * er, 0.f)) * ia);
+      const float dl = (-dyr >= dylo && -dyr <= dyhi)
+          ? -dxr : fmaf(rx, el, -__builtin_amdgcn_sqrtf(fmaxf(c0 - beta * el * el, 0.f)) * ia);
+      const float xa = mx - (dr + (0.25f + 1e-3f * fabsf(dr)));
+      const float xb = mx - (dl - (0.25f + 1e-3f * fabsf(dl)));
+      const int b0 = (ty - ly) * w - lx;
+      for (int tx = lx; tx < hx; ++tx) {
+        const float x0 = (float)(tx * TILE), x1 = (float)(min(tx * TILE + TILE, W) - 1);
+        if (x1 >= xa && x0 <= xb) m |= 1ull << (b0 + tx);
+      }
+    }
+  }
+  c = (uint32_t)__popcll(m);
+}
+
 __global__ __launch_bounds__(256) void k_count_live(
     const float2* __restrict__ means, const int32_t* __restrict__ radii,
     const float4* __restrict__ packed, int n, int gx, int gy, int W, int H,
@@ -570,50 +623,10 @@ __global__ __launch_bounds__(256) void k_count_live(
   const float2 mm = means[g];
   const float4 q0 = packed[(size_t)g * (REC / 4)];
   const float a11 = packed[(size_t)g * (REC / 4) + 1].x;
-  int lx = 0, ly = 0, hx = 0, hy = 0;
-  if (r > 0) get_rect(mm.x, mm.y, r, gx, gy, lx, ly, hx, hy);
-  const int w = hx - lx, area = (hx > lx && hy > ly) ? (hy - ly) * w : 0;
-  uint32_t c = 0;
-  uint64_t m = 0ull;
-  if (area > 64) {
-    c = (uint32_t)area;
-    m = ~0ull;
-  } else if (area > 0) {
-    const float mx = q0.x, my = q0.y, a00 = q0.z, a01 = q0.w;
-    const float beta = a00 * a11 - a01 * a01;
-    if (!(beta > 0.f) || !(a00 < 0.f) || !(a11 < 0.f) || !isfinite(mx) || !isfinite(my) ||
-        !isfinite(beta)) {
-      m = area == 64 ? ~0ull : (1ull << area) - 1ull;
-    } else {
-      // hardware reciprocals and square roots (1 ulp): far inside the widening below
-      const float thr = QSKIP - LIVE_MARGIN;
-      const float i00 = __builtin_amdgcn_rcpf(a00), ib = __builtin_amdgcn_rcpf(beta);
-      const float rx = -a01 * i00, ry = -a01 * __builtin_amdgcn_rcpf(a11), ia = -i00;
-      const float dxr = __builtin_amdgcn_sqrtf(fmaxf(thr * a11 * ib, 0.f));  // rightmost point
-      const float dyr = ry * dxr;
-      const float dymax = __builtin_amdgcn_sqrtf(fmaxf(a00 * thr * ib, 0.f));
-      const float c0 = a00 * thr;
-      for (int ty = ly; ty < hy; ++ty) {
-        const float dylo = my - (float)(min(ty * TILE + TILE, H) - 1), dyhi = my - (float)(ty * TILE);
-        if (dyhi < -dymax || dylo > dymax) continue;  // the row misses the ellipse
-        // dx = mx - x of the interval's ends: the extreme points if the row holds them,
-        // else the chord at the row edge nearest to them
-        const float er = fminf(fmaxf(dyr, dylo), dyhi), el = fminf(fmaxf(-dyr, dylo), dyhi);
-        const float dr = (dyr >= dylo && dyr <= dyhi)
-            ? dxr : fmaf(rx, er, __builtin_amdgcn_sqrtf(fmaxf(c0 - beta * er * er, 0.f)) * ia);
-        const float dl = (-dyr >= dylo && -dyr <= dyhi)
-            ? -dxr : fmaf(rx, el, -__builtin_amdgcn_sqrtf(fmaxf(c0 - beta * el * el, 0.f)) * ia);
-        const float xa = mx - (dr + (0.25f + 1e-3f * fabsf(dr)));
-        const float xb = mx - (dl - (0.25f + 1e-3f * fabsf(dl)));
-        const int b0 = (ty - ly) * w - lx;
-        for (int tx = lx; tx < hx; ++tx) {
-          const float x0 = (float)(tx * TILE), x1 = (float)(min(tx * TILE + TILE, W) - 1);
-          if (x1 >= xa && x0 <= xb) m |= 1ull << (b0 + tx);
-        }
-      }
-    }
-    c = (uint32_t)__popcll(m);
-  }
+  int lx, ly, w;
+  uint32_t c;
+  uint64_t m;
+  live_tiles(mm.x, mm.y, q0.z, q0.w, a11, r, gx, gy, W, H, lx, ly, w, c, m);
   counts[g] = c;
   rect[g] = make_int4(0, lx, ly, c ? w : 0);
   live[g] = m;
@@ -1295,18 +1308,30 @@ __device__ __forceinline__ void activate(const RawPtrs& R, int g, const Prologue
   a.area = fabsf(a.sx * a.sy);
 }
 
-template <int C>
+// Bins (LIVE = true: also the live count of k_count_live, from the values in registers;
+// counts / rect / live / misc as k_count_live writes them, bit for bit).
+struct LiveOut {
+  int gx, gy;
+  uint32_t* counts;
+  int4* rect;
+  uint64_t* live;
+  uint32_t* misc;
+};
+
+template <int C, bool LIVE>
 __global__ void k_prologue(RawPtrs R, Prologue pr, const float* __restrict__ kvec, int n,
                            float* __restrict__ means_out, int32_t* __restrict__ radii_out,
-                           float* __restrict__ packed) {
+                           float* __restrict__ packed, LiveOut lo) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (LIVE && g == 0) lo.misc[3] = 1u;
   if (g >= n) return;
   Act<C> a;
   activate<C>(R, g, pr, a);
   reinterpret_cast<float2*>(means_out)[g] = make_float2(a.mx, a.my);
   const float mid = 0.5f * (a.c00 + a.c11);
   const float disc = sqrtf(((a.c00 - a.c11) * 0.5f) * ((a.c00 - a.c11) * 0.5f) + a.c01 * a.c01);
-  radii_out[g] = (int32_t)ceilf(5.0f * sqrtf(mid + disc));
+  const int32_t rad = (int32_t)ceilf(5.0f * sqrtf(mid + disc));
+  radii_out[g] = rad;
   const float beta = 0.5f * a.curv * a.area * pr.dxy2;
   float r[REC];
   r[0] = a.mx;
@@ -1327,6 +1352,15 @@ __global__ void k_prologue(RawPtrs R, Prologue pr, const float* __restrict__ kve
   float4* dst = reinterpret_cast<float4*>(packed + (size_t)g * REC);
 #pragma unroll
   for (int i = 0; i < REC / 4; ++i) dst[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+  if (LIVE) {
+    int lx, ly, w;
+    uint32_t c;
+    uint64_t m;
+    live_tiles(r[0], r[1], r[2], r[3], r[4], rad, lo.gx, lo.gy, pr.W, pr.H, lx, ly, w, c, m);
+    lo.counts[g] = c;
+    lo.rect[g] = make_int4(0, lx, ly, c ? w : 0);
+    lo.live[g] = m;
+  }
 }
 
 __device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.0f; }
@@ -1881,14 +1915,48 @@ extern "C" int cvqpg_prologue_forward(const float* const* raw_params, int n, int
   hipStream_t s = (hipStream_t)stream;
   RasterWS w = carve(ws, n, capacity);
   const RawPtrs R = raw_ptrs(raw_params);
+  const LiveOut lo{0, 0, nullptr, nullptr, nullptr, nullptr};
   if (channels == 3)
-    hipLaunchKernelGGL(k_prologue<3>, dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k, n,
-                       means_out, radii_out, w.packed);
+    hipLaunchKernelGGL((k_prologue<3, false>), dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k,
+                       n, means_out, radii_out, w.packed, lo);
   else
-    hipLaunchKernelGGL(k_prologue<1>, dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k, n,
-                       means_out, radii_out, w.packed);
+    hipLaunchKernelGGL((k_prologue<1, false>), dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k,
+                       n, means_out, radii_out, w.packed, lo);
   CVQPG_LAUNCH_CHECK();
   return 0;
+}
+
+extern "C" int cvqpg_prologue_forward_live(const float* const* raw_params, int n, int channels,
+                                           int width, int height, const float* k, float dxy2,
+                                           float curv_lambda, int curv_mode, float curv_nyquist,
+                                           void* ws, size_t ws_bytes, int64_t capacity,
+                                           float* means_out, int32_t* radii_out, void* stream) {
+  if (int rc = check_common(n, channels, width, height, ws, ws_bytes, capacity)) return rc;
+  if (n == 0) return 0;
+  CVQPG_CHECK_ARG(raw_params && k && means_out && radii_out, "NULL argument pointer");
+  for (int i = 0; i < NPARAM; ++i) CVQPG_CHECK_ARG(raw_params[i], "raw parameter %d is NULL", i);
+  Prologue pr;
+  if (int rc = make_prologue(pr, width, height, dxy2, curv_lambda, curv_mode, curv_nyquist)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  RasterWS w = carve(ws, n, capacity);
+  const RawPtrs R = raw_ptrs(raw_params);
+  const LiveOut lo{ceil_div(width, TILE), ceil_div(height, TILE), w.counts, w.rect, w.live, w.misc};
+  if (channels == 3)
+    hipLaunchKernelGGL((k_prologue<3, true>), dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k,
+                       n, means_out, radii_out, w.packed, lo);
+  else
+    hipLaunchKernelGGL((k_prologue<1, true>), dim3(ceil_div(n, 256)), dim3(256), 0, s, R, pr, k,
+                       n, means_out, radii_out, w.packed, lo);
+  CVQPG_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int cvqpg_bin_scan(int n, void* ws, size_t ws_bytes, int64_t capacity,
+                              int32_t* num_pairs, void* stream) {
+  if (int rc = check_common(n, 1, 1, 1, ws, ws_bytes, capacity)) return rc;
+  RasterWS w = carve(ws, n, capacity);
+  return scan_exclusive(w.counts, w.offs, w.sums_n, n, w.misc, capacity, num_pairs, nullptr,
+                        (hipStream_t)stream);
 }
 
 extern "C" int cvqpg_raster_backward_raw(const float* const* raw_params, int n, int channels,

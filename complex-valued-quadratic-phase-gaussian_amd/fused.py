@@ -3,8 +3,8 @@
 One iteration of train_2d_gaussian_rgb.py:187-211 as a fixed sequence of libcvqpg calls
 on the current stream:
 
-    prologue      cvqpg_prologue_forward       activations, cov, inverse, radii, packing
-    binning       cvqpg_bin_count_live, cvqpg_bin_sort
+    prologue      cvqpg_prologue_forward_live  activations, cov, inverse, radii, packing, live count
+    binning       cvqpg_bin_scan, cvqpg_bin_sort
     raster        cvqpg_render_forward_packed  -> field [C,H,W]
     optics        cvqpg_blasm_forward          -> plane fields + intensities [D,C,H,W]
     loss          cvqpg_loss_forward_backward_stats  -> dL/dI, loss (target statistics
@@ -186,19 +186,19 @@ class FusedTrainer:
         return _lib.ptr(self.raster_ws), self.raster_ws.numel(), self.capacity
 
     def _prologue(self):
+        # the prologue fused with the live count (the raster kernels' own tile lists: keys
+        # below their skip threshold left out; results those of the full lists bit for bit)
         ws, nb, cap = self._ws()
-        _lib.check(_lib.lib().cvqpg_prologue_forward(
+        _lib.check(_lib.lib().cvqpg_prologue_forward_live(
             _lib.ptr_array(self.params), self.N, self.C, self.W, self.H, _lib.ptr(self.k),
             self.dxy2, self.curv_lambda, self.curv_mode, self.curv_nyquist, ws, nb, cap,
             _lib.ptr(self.means), _lib.ptr(self.radii), self._s()))
 
     def _count(self):
-        # the raster kernels' own tile lists (keys below their skip threshold left out; the
-        # step's results are those of the full lists bit for bit)
+        # the scan of the counts the prologue wrote
         ws, nb, cap = self._ws()
-        _lib.check(_lib.lib().cvqpg_bin_count_live(
-            _lib.ptr(self.means), _lib.ptr(self.radii), self.N, self.W, self.H, ws, nb, cap,
-            _lib.ptr(self.num_pairs), self._s()))
+        _lib.check(_lib.lib().cvqpg_bin_scan(self.N, ws, nb, cap, _lib.ptr(self.num_pairs),
+                                             self._s()))
 
     def _stages(self):
         L = _lib.lib()

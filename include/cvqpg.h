@@ -410,6 +410,25 @@ int cvqpg_prologue_forward(const float *const *raw_params, int n, int channels, 
                            int64_t capacity, float *means_out, int32_t *radii_out,
                            void *stream);
 
+/* cvqpg_prologue_forward that also does the live count of cvqpg_bin_count_live (from the
+ * values the prologue holds in registers; the same tiles, bit for bit), the prologue fused
+ * into the bin-count kernel.  No counterpart in the reference (its activations, radii and
+ * tile counts are separate torch launches and preprocessKernel, model_2d_gaussian_rgb.py:
+ * 83-153,424-439, cuda/forward.cu:15-26).  Follow it with cvqpg_bin_scan, then
+ * cvqpg_bin_sort. */
+int cvqpg_prologue_forward_live(const float *const *raw_params, int n, int channels, int width,
+                                int height, const float *k, float dxy2, float curv_lambda,
+                                int curv_mode, float curv_nyquist, void *ws, size_t ws_bytes,
+                                int64_t capacity, float *means_out, int32_t *radii_out,
+                                void *stream);
+
+/* The exclusive scan of the per-Gaussian tile counts already in ws (written by
+ * cvqpg_prologue_forward_live) and the key total: the second half of cvqpg_bin_count(_live),
+ * the reference's cub::DeviceScan::InclusiveSum (cuda/raster.cu:63-69); *num_pairs and the
+ * overflow flags as there. */
+int cvqpg_bin_scan(int n, void *ws, size_t ws_bytes, int64_t capacity, int32_t *num_pairs,
+                   void *stream);
+
 /* After cvqpg_render_backward_tiles: per-Gaussian sum of the partial records, adjoint of
  * the prologue, nan_to_num (train_2d_gaussian_rgb.py:206-208); writes the raw-parameter
  * gradients flat [(7 + 2C) N] in the order above and one sum of squares per 256

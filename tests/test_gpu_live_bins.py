@@ -1,9 +1,10 @@
-"""Live binning of the fused step (cvqpg_bin_count_live) against the reference's full tile
-lists.
+"""Live binning of the fused step (cvqpg_prologue_forward_live + cvqpg_bin_scan, and
+cvqpg_bin_count_live) against the reference's full tile lists.
 
 The fused step keys a tile of a Gaussian's 5-sigma rectangle only if the Gaussian's exponent
 reaches the raster kernels' skip threshold (2^-24 of a unit term) less a margin somewhere on
 the tile; both tile kernels skip every other key.  Checks:
+  * the prologue's fused live count gives the lists of cvqpg_bin_count_live bit for bit;
   * each tile's live list is an ordered subsequence of the reference's full list
     (oracle.binning, the reference's 64-bit key sort), and every key it leaves out belongs to
     a Gaussian whose FP64 exponent stays below 2^-24 on all 256 pixels of that tile;
@@ -55,6 +56,17 @@ def test_live_lists_are_the_kernels_lists(pkg, spec):
     assert 0 < P <= tr.capacity
     pl = tr.point_list[:P].cpu().numpy()
     rg = tr.ranges.cpu().numpy()
+    # the prologue's fused live count equals cvqpg_bin_count_live's, bit for bit
+    from cvqpg_amd import _lib
+    ws, nb, cap = tr._ws()
+    _lib.check(_lib.lib().cvqpg_bin_count_live(
+        _lib.ptr(tr.means), _lib.ptr(tr.radii), tr.N, tr.W, tr.H, ws, nb, cap,
+        _lib.ptr(tr.num_pairs), tr._s()))
+    st["bin_sort"]()
+    torch.cuda.synchronize()
+    assert int(tr.num_pairs.item()) == P
+    assert np.array_equal(tr.point_list[:P].cpu().numpy(), pl)
+    assert np.array_equal(tr.ranges.cpu().numpy(), rg)
     means = tr.means.cpu().numpy()
     radii = tr.radii.cpu().numpy()
     full = binning.bin_gaussians(means, radii, W, H)

@@ -60,6 +60,12 @@ def test_workspace_size_and_argument_errors(pkg):
     assert rc == 1 and b"workspace" in L.cvqpg_last_error()
     rc = L.cvqpg_bin_count_live(None, None, 10, 64, 64, None, 0, 0, None, None)
     assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+    rc = L.cvqpg_bin_scan(10, None, 0, 0, None, None)
+    assert rc == 1 and b"workspace" in L.cvqpg_last_error()
+    nb = L.cvqpg_raster_workspace_size(10, 3, 64, 64, 100)
+    rc = L.cvqpg_prologue_forward_live(None, 10, 3, 64, 64, None, 1e-10, 5e-7, 2, 0.9, 1, nb,
+                                       100, None, None, None)
+    assert rc == 1 and b"NULL" in L.cvqpg_last_error()
     # target window statistics: (D C) x (H - 10) x (W - 10) float2, checked before any launch
     assert L.cvqpg_loss_target_stats_size(3, 4, 40, 56) == 4 * 3 * 30 * 46 * 8
     assert L.cvqpg_loss_target_stats_size(3, 4, 10, 56) == 0
