@@ -27,6 +27,9 @@ SPEC_SMALL = dict(W=64, H=48, N=300, wavelengths=[639e-9, 515e-9, 473e-9], pitch
 # 32 x 32 = 1024 tiles: 10 key bits, two radix passes
 SPEC_512 = dict(W=512, H=512, N=20000, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
                 distances=[0.0, 2e-3], kappa=0.05)
+# partial tiles at the right (8 px) and bottom (6 px) edges
+SPEC_EDGE = dict(W=200, H=150, N=2000, wavelengths=[639e-9, 515e-9, 473e-9], pitch=8e-6,
+                 distances=[0.0, 3e-3], kappa=0.05)
 
 
 def trainer(spec, full=False, seed=0):
@@ -44,7 +47,7 @@ def trainer(spec, full=False, seed=0):
     return wl, tr
 
 
-@pytest.mark.parametrize("spec", [SPEC_SMALL, SPEC_512], ids=["64x48", "512x512"])
+@pytest.mark.parametrize("spec", [SPEC_SMALL, SPEC_EDGE, SPEC_512], ids=["64x48", "200x150", "512x512"])
 def test_live_lists_are_the_kernels_lists(pkg, spec):
     wl, tr = trainer(spec)
     st = dict(tr._stages())
@@ -107,9 +110,10 @@ def test_live_lists_are_the_kernels_lists(pkg, spec):
     assert worst < -24.0
 
 
-def test_live_step_equals_full_step(pkg):
-    wl_a, a = trainer(SPEC_512)
-    wl_b, b = trainer(SPEC_512, full=True)
+@pytest.mark.parametrize("spec", [SPEC_EDGE, SPEC_512], ids=["200x150", "512x512"])
+def test_live_step_equals_full_step(pkg, spec):
+    wl_a, a = trainer(spec)
+    wl_b, b = trainer(spec, full=True)
     for pa, pb in zip(a.params, b.params):
         assert torch.equal(pa, pb)
     for it in range(3):
