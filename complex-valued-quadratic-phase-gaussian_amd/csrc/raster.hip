@@ -806,6 +806,58 @@ __device__ __forceinline__ int row_span(float mx, float my, float a00, float a01
   return last < first ? 0 : first | ((last - first + 1) << 8);
 }
 
+// Column blocks (NCOL = TILE / BWD_LANES columns each) of the tile in which a Gaussian
+// reaches 2^QSKIP on the rows [yl, yh]: for each of the nx columns, the maximum of the
+// exponent over those rows (the concave quadratic's peak along the column, clamped to the
+// rows), the mirror of row_span.  Returned as first block | block count << 4 (the span's
+// bits 16 and up); 0 if no column reaches QSKIP.
+template <int NCOL>
+__device__ __forceinline__ int col_blocks(float mx, float my, float a00, float a01, float a11,
+                                          float x0, float yl, float yh, int nx) {
+  const float dylo = my - yh, dyhi = my - yl, ry = -a01 / a11;
+  int first = TILE, last = -1;
+#pragma unroll
+  for (int xx = 0; xx < TILE; ++xx) {
+    const float dx = mx - (x0 + (float)xx);
+    const float dy = fminf(fmaxf(ry * dx, dylo), dyhi);
+    const float q = fmaf(dy, fmaf(a11, dy, 2.f * a01 * dx), a00 * dx * dx);
+    if (xx < nx && q >= QSKIP) {
+      first = min(first, xx);
+      last = xx;
+    }
+  }
+  if (last < first) return 0;
+  const int b0 = first / NCOL;
+  return b0 | ((last / NCOL - b0 + 1) << 4);
+}
+
+// Lane layout of a queued key in the raster adjoint, from its span (first row | row count
+// << 8 | first column block << 16 | block count << 20).  By default lane cg of the key's
+// group covers column block cg on every one of the key's rows.  When the key's significant
+// columns lie in nb blocks and nb divides BWD_LANES (nb < BWD_LANES), the group's lanes
+// cover those nb blocks only, BWD_LANES / nb lanes per block taking every
+// (BWD_LANES / nb)-th row: block b0 + cg % nb, rows first + cg / nb, + stride, ...
+// A key with no significant column (nb = 0) walks no rows.
+struct KeyLanes {
+  int blk, row0, stride, trips;   // this lane's block, first row, row stride, row count
+};
+__device__ __forceinline__ KeyLanes key_lanes(int span, int cg) {
+  const int r0 = span & 0xff, nr = (span >> 8) & 0xff;
+  const int b0 = (span >> 16) & 0xf, nb = (span >> 20) & 0xf;
+  KeyLanes k{cg, r0, 1, nb == 0 ? 0 : nr};
+  if (nb > 0 && nb < BWD_LANES && BWD_LANES % nb == 0) {
+    k.stride = BWD_LANES / nb;
+    k.blk = b0 + cg % nb;
+    const int ro = cg / nb;
+    k.row0 = r0 + ro;
+    k.trips = nr > ro ? (nr - ro + k.stride - 1) / k.stride : 0;
+  }
+  return k;
+}
+
+// Row trips of a key's group: those of its lane 0, the most of any of its lanes.
+__device__ __forceinline__ int key_trips(int span) { return key_lanes(span, 0).trips; }
+
 // One tile of one hologram.  A key whose Gaussian stays below G = 2^-24 on every pixel of
 // the tile adds less than 2^-24 of a unit term to every adjoint sum (the reference
 // evaluates it, down to its clamp at e^-50): such keys get a zero partial record (so the
@@ -881,16 +933,16 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
     float K = 0.f, T2 = 0.f, T0[NCOL], T1[NCOL];
 #pragma unroll
     for (int j = 0; j < NCOL; ++j) T0[j] = T1[j] = 0.f;
-    const float dxc = mx - (float)(x0 + NCOL * cg);
-    const int r0 = span & 0xff, nr = span >> 8;
-    // the round is sorted by row count, so the wave's first key walks the most rows
-    const int nmax = __builtin_amdgcn_readfirstlane(nr);
+    const KeyLanes kl = key_lanes(span, cg);
+    const float dxc = mx - (float)(x0 + NCOL * kl.blk);
+    // the round is sorted by row trips, so the wave's first lane walks the most rows
+    const int nmax = __builtin_amdgcn_readfirstlane(kl.trips);
     for (int k = 0; k < nmax; ++k) {
-      if (k >= nr) continue;
-      const int yy = r0 + k;
+      if (k >= kl.trips) continue;
+      const int yy = kl.row0 + k * kl.stride;
       const float dy = my - (float)(y0 + yy);
       const float bq = 2.0f * a01 * dy, cq = a11 * dy * dy;
-      const float2* gr0 = &L.grad[0][yy * GROW + NCOL * cg];
+      const float2* gr0 = &L.grad[0][yy * GROW + NCOL * kl.blk];
 #pragma unroll
       for (int j = 0; j < NCOL; ++j) {
         const float dx = dxc - (float)j;
@@ -995,7 +1047,13 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       const float4 q0 = packed[(size_t)g * (REC / 4)];
       const float a11 = packed[(size_t)g * (REC / 4) + 1].x;
       keep = tile_qmax(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, fy1) >= QSKIP;
-      if (keep) span = row_span(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, ny);
+      if (keep) {
+        span = row_span(q0.x, q0.y, q0.z, q0.w, a11, fx0, fx1, fy0, ny);
+        const int r0 = span & 0xff, nr = span >> 8;
+        if (nr > 0)
+          span |= col_blocks<NCOL>(q0.x, q0.y, q0.z, q0.w, a11, fx0, fy0 + (float)r0,
+                                   fy0 + (float)(r0 + nr - 1), nx) << 16;
+      }
       if (!keep) {
         float4* dst = partials + (size_t)e * (REC / 4);
 #pragma unroll
@@ -1017,8 +1075,8 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
     const bool last = base + 256 >= rg.y;
     while (filled >= 256 || (last && filled > 0)) {
       const int n = min(filled, 256);
-      // counting sort of the round's n keys by row count, most rows first; the order
-      // within one count is that of the LDS atomics and does not matter
+      // counting sort of the round's n keys by row trips, most first; the order within
+      // one count is that of the LDS atomics and does not matter
       if (t <= TILE) L.hist[t] = 0;
       __syncthreads();
       int rk = 0, sp = 0, key = 0, er = 0;
@@ -1026,7 +1084,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
         key = L.keys[t];
         sp = L.span[t];
         er = L.rec[t];
-        rk = atomicAdd(&L.hist[TILE - (sp >> 8)], 1);
+        rk = atomicAdd(&L.hist[TILE - key_trips(sp)], 1);
       }
       __syncthreads();
       if (t == 0) {
@@ -1039,7 +1097,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
       }
       __syncthreads();
       if (t < n) {
-        const int at = L.hist[TILE - (sp >> 8)] + rk;
+        const int at = L.hist[TILE - key_trips(sp)] + rk;
         L.rkeys[at] = key;
         L.rspan[at] = sp;
         L.rrec[at] = er;
