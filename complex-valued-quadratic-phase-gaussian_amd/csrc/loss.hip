@@ -60,12 +60,39 @@ __device__ __forceinline__ void philox_round(uint32_t& c0, uint32_t& c1, uint32_
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
+// One tap of an 11-tap filter chain: the chain starts with a product (no zeroed
+// accumulator to set up) and continues with fused multiply-adds.  Every chain of the loss
+// and of the target statistics is formed this way, so the two agree bit for bit.
+__device__ __forceinline__ float tap(int b, float w, float x, float acc) {
+  return b == 0 ? w * x : fmaf(w, x, acc);
+}
+
+// Raw buffer view of one plane (bytes): a load at an offset at or past the end returns 0,
+// so a guarded load is one load from a selected offset, with no branch, no zeroed
+// register and no 64-bit address arithmetic.  Word 3 is the gfx9-family raw-buffer format.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_view(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+constexpr uint32_t OUTSIDE = 0x80000000u;   // an offset past every plane
+__device__ __forceinline__ float load_f32(__amdgpu_buffer_rsrc_t v, uint32_t off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(v, (int)off, 0, 0));
+}
+__device__ __forceinline__ float2 load_f32x2(__amdgpu_buffer_rsrc_t v, uint32_t off) {
+  const auto u = __builtin_amdgcn_raw_buffer_load_b64(v, (int)off, 0, 0);
+  return make_float2(__uint_as_float(u[0]), __uint_as_float(u[1]));
+}
+
 // target noise: one Philox4x32-10 draw gives the four normals of pixels (y, x) with y in
 // 4 g .. 4 g + 3 (two Box-Muller pairs); the counter is (g W + x, plane, step).
 __device__ __forceinline__ void normals4(unsigned long long seed, uint32_t step, uint32_t plane,
                                          uint32_t idx, float (&z)[4]) {
   uint32_t c0 = idx, c1 = plane, c2 = step, c3 = 0x5EED0001u;
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  // the key schedule is formed here, on the scalar unit, at every draw: hoisted out of the
+  // streaming loop its twenty round keys did not fit the scalar registers and were kept in
+  // vector-register lanes, read back one by one
+  asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     philox_round(c0, c1, c2, c3, k0, k1);
@@ -178,26 +205,38 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
   const bool two = i < WIN - 1;
   float pa0 = 0.f, pb0 = 0.f, pa1 = 0.f, pb1 = 0.f;   // next input row (raw inten, target)
   float qM = 0.f, qN = 0.f;                           // next pixel row values
+  // out-of-range rows and columns read 0 (buffer loads past the plane's end)
+  const uint32_t pbytes = (uint32_t)(P.H * P.W) * 4u;
+  const __amdgpu_buffer_rsrc_t vin = plane_view(inp, pbytes), vtg = plane_view(tgp, pbytes);
+  const __amdgpu_buffer_rsrc_t vmk = plane_view(mkp, pbytes);
+  const __amdgpu_buffer_rsrc_t vnz = plane_view(nzp ? nzp : inp, nzp ? pbytes : 0u);
   auto load_row = [&](int y) {
-    pa0 = pb0 = pa1 = pb1 = 0.f;
-    if (y >= 0 && y < P.H) {
-      const int r = y * Wd;
-      if (gx0 >= 0 && gx0 < Wd) { pa0 = inp[r + gx0]; pb0 = tgp[r + gx0]; }
-      if (two && gx1 < Wd) { pa1 = inp[r + gx1]; pb1 = tgp[r + gx1]; }
+    const bool yin = y >= 0 && y < P.H;
+    const uint32_t r = (uint32_t)(y * Wd);
+    const uint32_t o0 = yin && gx0 >= 0 && gx0 < Wd ? (r + (uint32_t)gx0) * 4u : OUTSIDE;
+    pa0 = load_f32(vin, o0);
+    pb0 = load_f32(vtg, o0);
+    if (two) {
+      const uint32_t o1 = yin && gx1 < Wd ? (r + (uint32_t)gx1) * 4u : OUTSIDE;
+      pa1 = load_f32(vin, o1);
+      pb1 = load_f32(vtg, o1);
     }
   };
   auto load_pix = [&](int pr) {
-    if (pix_col && pr >= p0 && pr < p_end) {
-      const int r = pr * Wd + mc;
-      qM = mkp[r];
-      if (nzp) qN = nzp[r];
-    }
+    const uint32_t o = pix_col && pr >= p0 && pr < p_end ? (uint32_t)(pr * Wd + mc) * 4u : OUTSIDE;
+    qM = load_f32(vmk, o);
+    if (nzp) qN = load_f32(vnz, o);
   };
   // target statistics of map row q at this thread's map column (TS)
   const float2* tsp = TS ? tstats + (size_t)dc * P.Hv * P.Wv : nullptr;
+  const __amdgpu_buffer_rsrc_t vst =
+      plane_view(TS ? (const void*)tsp : (const void*)inp, TS ? (uint32_t)(P.Hv * P.Wv) * 8u : 0u);
   float2 sN = make_float2(0.f, 0.f);
   auto load_stats = [&](int q) {
-    if (TS && q >= 0 && q < P.Hv && map_col) sN = tsp[q * P.Wv + mc];
+    if (TS) {
+      const uint32_t o = q >= 0 && q < P.Hv && map_col ? (uint32_t)(q * P.Wv + mc) * 8u : OUTSIDE;
+      sN = load_f32x2(vst, o);
+    }
   };
   float hm[WIN][NM];   // moment ring: row-filtered moments of input rows
 #pragma unroll
@@ -253,12 +292,10 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       {
         float m[NM];
 #pragma unroll
-        for (int j = 0; j < NM; ++j) m[j] = 0.f;
-#pragma unroll
         for (int b = 0; b < WIN; ++b) {
           const float w = P.win[b];
 #pragma unroll
-          for (int j = 0; j < NM; ++j) m[j] = fmaf(w, smo[buf][j][i + b], m[j]);
+          for (int j = 0; j < NM; ++j) m[j] = tap(b, w, smo[buf][j][i + b], m[j]);
         }
 #pragma unroll
         for (int j = 0; j < NM; ++j) hm[k][j] = m[j];
@@ -273,12 +310,12 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
         for (int a = 0; a < WIN; ++a) {
           const int sl = (k + 1 + a) % WIN;
           const float w = P.win[a];
-          mu1 = fmaf(w, hm[sl][0], mu1);
-          e11 = fmaf(w, hm[sl][1], e11);
-          e12 = fmaf(w, hm[sl][2], e12);
+          mu1 = tap(a, w, hm[sl][0], mu1);
+          e11 = tap(a, w, hm[sl][1], e11);
+          e12 = tap(a, w, hm[sl][2], e12);
           if (!TS) {
-            mu2 = fmaf(w, hm[sl][NM - 2], mu2);
-            e22 = fmaf(w, hm[sl][NM - 1], e22);
+            mu2 = tap(a, w, hm[sl][NM - 2], mu2);
+            e22 = tap(a, w, hm[sl][NM - 1], e22);
           }
         }
         if (TS) {
@@ -312,9 +349,9 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
 #pragma unroll
         for (int b = 0; b < WIN; ++b) {
           const float w = P.win[b];
-          g0 = fmaf(w, smap[buf][0][i - b], g0);
-          g1 = fmaf(w, smap[buf][1][i - b], g1);
-          g2 = fmaf(w, smap[buf][2][i - b], g2);
+          g0 = tap(b, w, smap[buf][0][i - b], g0);
+          g1 = tap(b, w, smap[buf][1][i - b], g1);
+          g2 = tap(b, w, smap[buf][2][i - b], g2);
         }
         gring[k][0] = g0; gring[k][1] = g1; gring[k][2] = g2;
       }
@@ -332,9 +369,9 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
       for (int a = 0; a < WIN; ++a) {
         const int sl = (k - a + WIN) % WIN;
         const float w = P.win[a];
-        gA = fmaf(w, gring[sl][0], gA);
-        gB = fmaf(w, gring[sl][1], gB);
-        gC = fmaf(w, gring[sl][2], gC);
+        gA = tap(a, w, gring[sl][0], gA);
+        gB = tap(a, w, gring[sl][1], gB);
+        gC = tap(a, w, gring[sl][2], gC);
       }
       const float I = clamp01(cI);
       float gM = 0.f;
@@ -385,17 +422,18 @@ __global__ __launch_bounds__(256) void k_target_stats(const float* __restrict__ 
   if (mc >= P.Wv) return;
   const float* tp = target + (size_t)dc * P.H * P.W;
   float mu = 0.f, e = 0.f;
+#pragma unroll
   for (int a = 0; a < WIN; ++a) {
     const float* row = tp + (size_t)(q + a) * P.W + mc;
     float h1 = 0.f, h2 = 0.f;
 #pragma unroll
     for (int b = 0; b < WIN; ++b) {
       const float t = row[b];
-      h1 = fmaf(P.win[b], t, h1);
-      h2 = fmaf(P.win[b], t * t, h2);
+      h1 = tap(b, P.win[b], t, h1);
+      h2 = tap(b, P.win[b], t * t, h2);
     }
-    mu = fmaf(P.win[a], h1, mu);
-    e = fmaf(P.win[a], h2, e);
+    mu = tap(a, P.win[a], h1, mu);
+    e = tap(a, P.win[a], h2, e);
   }
   stats[((size_t)dc * P.Hv + q) * P.Wv + mc] = make_float2(mu, e);
 }
