@@ -412,9 +412,9 @@ __global__ __launch_bounds__(NTH, CVQPG_LOSS_WPE) void k_loss_stream(
 }
 
 // Window statistics of the target for k_loss_stream<true>: (mu_t, E[t^2]) of every valid
-// map entry [D C][Hv][Wv], with the streaming kernel's FMA chains (row filter over columns
-// mc .. mc + 10, then column filter over rows q .. q + 10), so that the loss and its
-// gradient are bit-identical to filtering the target in the streaming kernel.
+// map entry [D C][Hv][Wv], with the streaming kernel's filter chains (tap; row filter over
+// columns mc .. mc + 10, then column filter over rows q .. q + 10), so that the loss and
+// its gradient are bit-identical to filtering the target in the streaming kernel.
 // grid (ceil(Wv / 256), Hv, D C)
 __global__ __launch_bounds__(256) void k_target_stats(const float* __restrict__ target,
                                                       LossParams P, float2* __restrict__ stats) {
