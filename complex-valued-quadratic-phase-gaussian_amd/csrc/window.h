@@ -33,13 +33,19 @@ __device__ __forceinline__ float2 glue_adjoint(float2 h, float2 g) {
   return make_float2(-q * hy, q * hx);         // q * i * hh
 }
 
+// Tukey ramps 0.5 (1 - cos(pi k / n)) and 0.5 (1 + cos(pi k / n)) in their half-angle forms
+// sin^2(pi k / 2n) and cos^2(pi k / 2n), which do not cancel near the start of a ramp, on the
+// sine units (revolutions: k / 4n in [0, 1/4], no range reduction)
 __device__ __forceinline__ float taper(int t, int total, int inner_start, int inner_size) {
-  if (t < inner_start)
-    return 0.5f * (1.0f - cosf(3.14159265358979f * (float)(t + 1) / (float)inner_start));
+  if (t < inner_start) {
+    const float s = __builtin_amdgcn_sinf(0.25f * (float)(t + 1) / (float)inner_start);
+    return s * s;
+  }
   const int rs = inner_start + inner_size;
   if (t >= rs) {
     const int n_right = total - rs;
-    return 0.5f * (1.0f + cosf(3.14159265358979f * (float)(t - rs + 1) / (float)n_right));
+    const float c = __builtin_amdgcn_cosf(0.25f * (float)(t - rs + 1) / (float)n_right);
+    return c * c;
   }
   return 1.0f;
 }
