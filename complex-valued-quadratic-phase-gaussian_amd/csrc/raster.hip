@@ -69,8 +69,13 @@ constexpr float REV_DIRECT = 250.0f;
 __device__ __forceinline__ bool rev_direct(float u, float w) {
   return fabsf(u) + (-QSKIP) * fabsf(w) <= REV_DIRECT;
 }
+// FR = false: the phase of a Gaussian that allows it (all of a batch or wave do) goes to the
+// units unreduced; FR = true: each Gaussian keeps its own choice (direct), so a term's bits
+// depend only on its own Gaussian, not on the batch or wave it was evaluated in.
 template <bool FR>
-__device__ __forceinline__ float rev_of(float x) { return FR ? fract_f(x) : x; }
+__device__ __forceinline__ float rev_of(float x, bool direct) {
+  return FR && !direct ? fract_f(x) : x;
+}
 
 // ------------------------------------------------------------------------------
 // workspace
@@ -697,7 +702,8 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
     }
     const uint64_t bal = __ballot(keep);
     if (lane == 0) s_cnt[wave] = __popcll(bal);
-    // the batch's phases go to the sine units unreduced when every kept Gaussian allows it
+    // the batch's phases go to the sine units unreduced when every kept Gaussian allows it;
+    // otherwise each Gaussian follows its own choice
     const bool reduce = __syncthreads_or(keep && !direct) != 0;
     int slot = __popcll(bal & ((1ull << lane) - 1ull));
     for (int w = 0; w < wave; ++w) slot += s_cnt[w];
@@ -709,10 +715,11 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
         sg[4 * slot + 0] = q0;
         sg[4 * slot + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
         sg[4 * slot + 2] = make_float4(q2.z, q2.w, q3.x * o, q3.y);
-        sg[4 * slot + 3] = make_float4(q3.z, q3.w * o, 0.f, 0.f);
+        sg[4 * slot + 3] = make_float4(q3.z, q3.w * o, direct ? 1.f : 0.f, 0.f);
       } else {
         sg[4 * slot + 0] = q0;
         sg[4 * slot + 1] = make_float4(q1.x, q1.w, q2.x, q2.y * o);
+        sg[4 * slot + 3] = make_float4(0.f, 0.f, direct ? 1.f : 0.f, 0.f);
       }
     }
     __syncthreads();
@@ -731,8 +738,10 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
       // pixels beyond the clamp get 2^q < e^-50 instead of the clamped value
       if (__ballot(q >= QSKIP) == 0ull) continue;
       const float G = fast_exp2(q);
+      // the Gaussian's own choice, read only in a batch that mixes both kinds
+      const bool dj = FR ? sg[4 * j + 3].z != 0.f : true;
       {
-        const float rev = rev_of<FR>(fmaf(p1.z, q, p1.y));
+        const float rev = rev_of<FR>(fmaf(p1.z, q, p1.y), dj);
         const float a = p1.w * G;
         ar[0] = fmaf(a, cos_rev(rev), ar[0]);
         ai[0] = fmaf(a, sin_rev(rev), ai[0]);
@@ -740,11 +749,11 @@ __device__ __forceinline__ void render_fwd_tile(int tile, const int2* __restrict
       if (C == 3) {
         const float4 p2 = sg[4 * j + 2];
         const float4 p3 = sg[4 * j + 3];
-        const float rev1 = rev_of<FR>(fmaf(p2.y, q, p2.x));
+        const float rev1 = rev_of<FR>(fmaf(p2.y, q, p2.x), dj);
         const float a1 = p2.z * G;
         ar[1] = fmaf(a1, cos_rev(rev1), ar[1]);
         ai[1] = fmaf(a1, sin_rev(rev1), ai[1]);
-        const float rev2 = rev_of<FR>(fmaf(p3.x, q, p2.w));
+        const float rev2 = rev_of<FR>(fmaf(p3.x, q, p2.w), dj);
         const float a2 = p3.y * G;
         ar[C - 1] = fmaf(a2, cos_rev(rev2), ar[C - 1]);
         ai[C - 1] = fmaf(a2, sin_rev(rev2), ai[C - 1]);
@@ -966,7 +975,8 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
     bool direct = true;
 #pragma unroll
     for (int c = 0; c < C; ++c) direct = direct && rev_direct(u[c], w[c]);
-    // the wave's phases go to the sine units unreduced when all its keys allow it
+    // the wave's phases go to the sine units unreduced when all its keys allow it;
+    // otherwise each key follows its own choice
     const bool reduce = __ballot(valid && !direct) != 0ull;
     auto walk = [&](auto fr) {
     constexpr bool FR = decltype(fr)::value;
@@ -985,7 +995,7 @@ __device__ __forceinline__ void render_bwd_tile(int tile, const int2* __restrict
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float2 gg = gr0[c * TILE * GROW + j];
-          const float rev = rev_of<FR>(fmaf(w[c], q, u[c]));
+          const float rev = rev_of<FR>(fmaf(w[c], q, u[c]), direct);
           const float sn = sin_rev(rev), cs = cos_rev(rev);
           const float Pc = fmaf(gg.x, cs, gg.y * sn);
           const float Qc = fmaf(gg.y, cs, -gg.x * sn);

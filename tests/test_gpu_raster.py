@@ -183,6 +183,57 @@ def test_large_phase_forward_and_backward(pkg, share, seed):
         assert e <= 7e-4, (k, e)
 
 
+def _render(d, W, H, leaves=False):
+    from cvqpg_amd import qpf_render
+    g = to_dev(d)
+    args = [g[k].clone().requires_grad_() if leaves else g[k] for k in NAMES]
+    return qpf_render(*args, g["radii"], g["k"], d["dxy2"], W, H), args
+
+
+def _concat(a, b, front=False):
+    first, second = (b, a) if front else (a, b)
+    out = {k: np.ascontiguousarray(np.concatenate([first[k], second[k]])) for k in NAMES + ("radii",)}
+    out["k"], out["dxy2"] = a["k"], a["dxy2"]
+    return out
+
+
+def test_terms_independent_of_batch_and_wave(pkg):
+    """A Gaussian's forward terms and adjoint records depend on the Gaussian alone, also when
+    large-phase Gaussians (reduced phase) share its forward batch or adjoint wave with
+    Gaussians whose phase goes to the sine units unreduced.
+      forward: decoys in front of the list (keyed on every tile, reaching 2^-24 on none, so
+        dropped when a batch is staged) shift every tile's 256-key batch boundaries: the
+        field must stay bit for bit the same;
+      adjoint: large-phase Gaussians appended to the list change the rounds and waves of the
+        others' keys: with the same upstream gradient, the others' gradients must stay bit
+        for bit the same."""
+    n, W, H, C = 1500, 48, 48, 3       # 739-1262 keys per tile: several batches and rounds
+    d, nbig = large_phase_inputs(n, W, H, C, 41, 0.004)
+    base, _ = _render(d, W, H)
+    for shift in (1, 3, 7):
+        decoy = {k: np.repeat(v[:1], shift, axis=0) for k, v in d.items() if k in NAMES}
+        decoy["means"] = np.full((shift, 2), -300.0, np.float32)
+        decoy["inv_cov"] = np.tile(np.array([[1.0, 0.0, 1.0]], np.float32), (shift, 1))
+        decoy["radii"] = np.full(shift, 2000, np.int32)
+        out, _ = _render(_concat(d, decoy, front=True), W, H)
+        print(f"forward, {nbig} large-phase Gaussians, {shift} decoys in front: "
+              f"{int((out != base).sum())} field values differ", flush=True)
+        assert torch.equal(out, base)
+    rng = np.random.default_rng(42)
+    gr = torch.complex(torch.as_tensor(rng.normal(size=(C, H, W))),
+                       torch.as_tensor(rng.normal(size=(C, H, W)))).to(torch.complex64).to(DEV)
+    out, leaves = _render(d, W, H, leaves=True)
+    out.backward(gr)
+    ref = [t.grad.clone() for t in leaves]
+    extra, _ = large_phase_inputs(60, W, H, C, 43, 1.0)
+    out2, leaves2 = _render(_concat(d, extra), W, H, leaves=True)
+    out2.backward(gr)
+    for name, r, t in zip(NAMES, ref, leaves2):
+        diff = int((t.grad[:n] != r).sum())
+        print(f"adjoint, 60 large-phase Gaussians appended: {name} {diff} values differ", flush=True)
+        assert torch.equal(t.grad[:n], r), name
+
+
 def test_deterministic(pkg):
     from cvqpg_amd import qpf_render
     n, W, H, C = 2000, 200, 120, 3
