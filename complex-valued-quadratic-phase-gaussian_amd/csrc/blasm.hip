@@ -86,7 +86,8 @@ struct Optics {
   float fxmax[MAXD][3];  // band limit on |FX| (column frequency)
   float fymax[MAXD][3];  // band limit on |FY| (row frequency)
   int ap_open;
-  int band_active;       // 0 when no band limit and no evanescence can occur on the grid
+  int band_active;       // 0 when no band limit and no evanescence can occur on the grid and
+                         // every residual phase (z / lambda) g is within 250 revolutions
   float ap_r2, apr0, aprs, apc0, apcs;
   const float* ap_tab;   // custom aperture [nu][nv] on the centred grid, or NULL
 };
@@ -242,8 +243,10 @@ __device__ __forceinline__ float tf_point(const Optics& o, int c, int kr, int kc
 }
 
 // H_d / carrier at a point: exp(-i 2 pi (z/lambda) g) inside the band, 0 outside.
-// BAND = false compiles the point without the band-limit test (no band limit and no
-// evanescence can occur on the grid: Optics::band_active == 0).
+// BAND = false compiles the point without the band-limit test and hands the residual phase
+// to the sine units unreduced (they take revolutions over [-256, 256]); it runs only when no
+// band limit and no evanescence can occur on the grid and every residual phase lies within
+// 250 revolutions (Optics::band_active == 0).
 template <bool BAND = true>
 __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int d, int kr, int kc) {
   if (g < 0.0f) return make_float2(0.f, 0.f);
@@ -253,7 +256,8 @@ __device__ __forceinline__ float2 tf_value(const Optics& o, float g, int c, int 
     tf_freqs(o, kr, kc, ir, ic, fy, fx);
     if (!(fabsf(fx) < o.fxmax[d][c]) || !(fabsf(fy) < o.fymax[d][c])) return make_float2(0.f, 0.f);
   }
-  const float fr = __builtin_amdgcn_fractf(-o.zl[d][c] * g);
+  const float ph = -o.zl[d][c] * g;
+  const float fr = BAND ? __builtin_amdgcn_fractf(ph) : ph;
   return make_float2(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
 }
 
@@ -1510,10 +1514,16 @@ static int make_optics(Optics& o, int C, int H, int W, int pH, int pW, int D,
     o.band_active = 0;
     for (int c = 0; c < C; ++c) {
       const double lam = wavelengths[c];
-      if (lam * lam * (fxmaxabs * fxmaxabs + fymaxabs * fymaxabs) >= 1.0) o.band_active = 1;
-      for (int d = 0; d < D; ++d)
+      const double x = lam * lam * (fxmaxabs * fxmaxabs + fymaxabs * fymaxabs);
+      if (x >= 1.0) o.band_active = 1;
+      // the largest residual phase on the grid, (z / lambda) g with g = x / (1 + sqrt(1 - x))
+      // increasing in x: past 250 revolutions the kernels reduce it before the sine units
+      const double gmax = x < 1.0 ? x / (1.0 + sqrt(1.0 - x)) : 1.0;
+      for (int d = 0; d < D; ++d) {
         if (!((float)fxmaxabs < o.fxmax[d][c]) || !((float)fymaxabs < o.fymax[d][c]))
           o.band_active = 1;
+        if (fabs(distances[d]) / lam * gmax > 250.0) o.band_active = 1;
+      }
     }
   }
   // circular aperture on the centred grid: linspace(-n/2, n/2, n) per axis
