@@ -658,7 +658,9 @@ __global__ __launch_bounds__(256) void k_count_live(
 // ------------------------------------------------------------------------------
 // raster forward
 // ------------------------------------------------------------------------------
-// LDS record per Gaussian (16 floats): mx my a00 a01 | a11 u0 w0 amp0 | u1 w1 amp1 u2 | w2 amp2 - -
+// LDS record per Gaussian (16 floats): mx my a00 a01 | a11 u0 w0 amp0 | u1 w1 amp1 u2 |
+// w2 amp2 direct -, with direct = 1 when the Gaussian's phases go to the sine units
+// unreduced (C = 1: the first two float4s, and direct in the fourth)
 // One 16x16 tile of one hologram; sg is the workgroup's 16 KiB staging buffer.  Every
 // write to sg follows a workgroup barrier inside this function, so consecutive calls by
 // one workgroup need no barrier between them.
